@@ -1,0 +1,206 @@
+/*
+ * sph.h - C ABI of the MI355X SPH solver (libnprsph.so).
+ *
+ * Drop-in replacement for the three GL compute programs of NPR-SPH
+ * (rho_pres_comp.glsl, force_comp.glsl, integrate_comp.glsl) and the host
+ * code that owns their buffers and dispatches them (NPR-SPH/Main.cpp).
+ * Plain C: no HIP, torch or C++ types cross this boundary.  Streams are
+ * passed as `void*` (a hipStream_t, or NULL to let the library create one).
+ *
+ * Conventions (mirroring NPR-SPH/InitShader.cpp:47-112 and
+ * NPR-SPH/Main.cpp:428-451, where a failed shader build returns (GLuint)-1,
+ * logs the reason and leaves the previous program in use):
+ *   - every int-returning call returns SPH_OK (0) or a negative SPH_ERR_*;
+ *   - on failure the context stays valid and sph_last_error() holds the
+ *     reason (sph_create_error() for failures of the create calls);
+ *   - calls are asynchronous on the context's stream; only sph_download,
+ *     sph_download_grid, sph_metrics and sph_synchronize wait for it;
+ *   - one context per stream; a context is not thread-safe.
+ */
+#ifndef NPR_SPH_AMD_SPH_H
+#define NPR_SPH_AMD_SPH_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SPH_ABI_VERSION 1
+
+enum {
+    SPH_OK = 0,
+    SPH_ERR_INVALID = -1, /* bad argument: null pointer, size, scene index, non-finite constant */
+    SPH_ERR_HIP = -2,     /* a HIP runtime call failed; message holds hipGetErrorString */
+    SPH_ERR_NOMEM = -3,   /* device or host allocation failed */
+    SPH_ERR_STATE = -4    /* call not valid in the context's current state */
+};
+
+/* One particle, 64 bytes, std430 layout of the SSBO at binding 0.
+ * Replaces `struct Particle` of NPR-SPH/Main.cpp:93-99 and
+ * rho_pres_comp.glsl:12-18 / force_comp.glsl:12-18 / integrate_comp.glsl:14-20.
+ * extras = (rho, pressure, age, unused); pos.w, vel.w, force.w, extras.z and
+ * extras.w are never written by the solver. */
+typedef struct sph_particle {
+    float pos[4];
+    float vel[4];
+    float force[4];
+    float extras[4];
+} sph_particle_t;
+
+/* ConstantsUniform, NPR-SPH/Main.cpp:110-116 (std140 block at binding 1,
+ * rho_pres_comp.glsl:25-31).  Defaults: {0.02, 4.0, 3000, 1000}. */
+typedef struct sph_constants {
+    float mass;
+    float smoothing_coeff;
+    float visc;
+    float resting_rho;
+} sph_constants_t;
+
+/* BoundaryUniform, NPR-SPH/Main.cpp:118-122 (std140 block at binding 2,
+ * integrate_comp.glsl:27-31).  Defaults: upper (0.5,1,0.5,1), lower
+ * (-0.1,-0.35,-0.1,1).  The .w lanes are ignored. */
+typedef struct sph_boundary {
+    float upper[4];
+    float lower[4];
+} sph_boundary_t;
+
+/* Constants the reference bakes into its shaders:
+ * PARTICLE_RADIUS 0.005 (rho_pres_comp.glsl:5), GAS_CONST 2000
+ * (rho_pres_comp.glsl:33), G = (0,-9806.65,0) (force_comp.glsl:33),
+ * DAMPING 0.3 (integrate_comp.glsl:8), dt = 1/NUM_PARTICLES = 1e-4
+ * (integrate_comp.glsl:33; kept at 1e-4 for every particle count). */
+typedef struct sph_fixed {
+    float particle_radius;
+    float gas_const;
+    float gravity[3];
+    float damping;
+    float dt;
+} sph_fixed_t;
+
+/* Seeded jittered lattice (device-side scene initialiser, replaces
+ * make_grid/init_particles of NPR-SPH/Main.cpp:488-539).  Particle
+ * k = (ix*ny + iy)*nz + iz (x outermost, z innermost, as make_grid) sits at
+ * origin + i*spacing + jitter, jitter per axis uniform in [-jitter, +jitter)
+ * from a counter-based SplitMix64 stream keyed by (seed, 3k+axis).
+ * Only the first `count` lattice sites are used (count <= nx*ny*nz).
+ * vel = force = extras = 0, pos.w = 1. */
+typedef struct sph_lattice {
+    int32_t nx, ny, nz;
+    uint32_t count;
+    float origin[3];
+    float spacing;
+    float jitter;
+    uint64_t seed;
+} sph_lattice_t;
+
+/* Geometry of one scene's uniform neighbour grid (read-only diagnostics). */
+typedef struct sph_grid_info {
+    float lo[3];          /* grid origin = boundary lower corner */
+    float inv_cell;       /* 1 / cell edge (float32, used by the cell hash) */
+    double cell;          /* cell edge (>= smoothing length) */
+    int32_t dims[3];      /* cells per axis */
+    uint32_t ncells;      /* dims[0]*dims[1]*dims[2]; cell `ncells` holds NaN positions */
+    uint32_t cell_base;   /* global index of this scene's first cell */
+    uint32_t first;       /* first particle of this scene in the AoS buffer */
+    uint32_t count;       /* particles in this scene */
+} sph_grid_info_t;
+
+/* Per-scene reductions over the current state (sph_metrics). */
+typedef struct sph_metrics {
+    double mean_rho;      /* mean of extras.x */
+    double kinetic;       /* sum of 0.5*mass*|vel|^2 */
+    double com[3];        /* mean position */
+    double max_speed;     /* max |vel| */
+    uint64_t nonfinite;   /* particles with a non-finite pos or vel component */
+} sph_metrics_t;
+
+typedef struct sph_ctx sph_ctx;
+
+/* Reference defaults (Main.cpp:110-122 and the shader-baked constants). */
+void sph_defaults(sph_constants_t* c, sph_boundary_t* b, sph_fixed_t* f);
+
+/* Create a single-scene solver for n particles (n >= 1).  Any of c/b/f may be
+ * NULL for the defaults.  The AoS buffer starts zeroed except pos.w = 1.
+ * Replaces initOpenGL's buffer setup (Main.cpp:524-527, 632-641). */
+int sph_create(sph_ctx** out, size_t n, const sph_constants_t* c, const sph_boundary_t* b,
+               const sph_fixed_t* f, void* stream);
+
+/* Create a solver holding `nscenes` independent scenes of counts[s] particles,
+ * stored back to back in one AoS buffer (scene s starts at sum(counts[<s])).
+ * Every pass advances all scenes in one launch.  c/b/f are arrays of
+ * nscenes entries or NULL for the defaults.  No reference counterpart: the
+ * reference runs one scene. */
+int sph_create_batch(sph_ctx** out, int nscenes, const size_t* counts, const sph_constants_t* c,
+                     const sph_boundary_t* b, const sph_fixed_t* f, void* stream);
+
+/* Message of the last failed sph_create / sph_create_batch on this thread. */
+const char* sph_create_error(void);
+
+void sph_destroy(sph_ctx* ctx);
+const char* sph_last_error(const sph_ctx* ctx);
+
+size_t sph_size(const sph_ctx* ctx);     /* total particles */
+int sph_num_scenes(const sph_ctx* ctx);
+
+/* Parameter upload (sendUniforms, Main.cpp:254-282).  Takes effect for every
+ * pass enqueued after the call; scene = -1 applies to all scenes.  A change of
+ * smoothing length or box that needs a different grid is applied here too. */
+int sph_set_constants(sph_ctx* ctx, int scene, const sph_constants_t* c);
+int sph_set_boundary(sph_ctx* ctx, int scene, const sph_boundary_t* b);
+int sph_set_fixed(sph_ctx* ctx, int scene, const sph_fixed_t* f);
+int sph_get_params(const sph_ctx* ctx, int scene, sph_constants_t* c, sph_boundary_t* b, sph_fixed_t* f);
+
+/* Host <-> device copies of particles [first, first+n) of the AoS buffer
+ * (glBufferData of init_particles, Main.cpp:526).  sph_download waits. */
+int sph_upload(sph_ctx* ctx, const sph_particle_t* host, size_t first, size_t n);
+int sph_download(sph_ctx* ctx, sph_particle_t* host, size_t first, size_t n);
+
+/* Device-side scene initialiser: fills scene `scene` from a lattice spec. */
+int sph_init_lattice(sph_ctx* ctx, int scene, const sph_lattice_t* spec);
+
+/* The three-pass surface (Main.cpp:295-303).  Each pass reads and writes the
+ * AoS buffer exactly as its shader does:
+ *   sph_rho_pres : extras.x = rho, extras.y = p     (rho_pres_comp.glsl:35-58)
+ *   sph_force    : force.xyz                        (force_comp.glsl:35-66)
+ *   sph_integrate: vel.xyz, pos.xyz                 (integrate_comp.glsl:35-81)
+ * Stream order replaces glMemoryBarrier. */
+int sph_rho_pres(sph_ctx* ctx);
+int sph_force(sph_ctx* ctx);
+int sph_integrate(sph_ctx* ctx);
+
+/* nsteps x (rho_pres, force, integrate), the per-frame block of
+ * Main.cpp:291-304, run as captured HIP graphs on cell-sorted state.  The AoS
+ * buffer afterwards equals what the three passes would leave, bit for bit. */
+int sph_step(sph_ctx* ctx, int nsteps);
+
+/* Device pointer of the AoS buffer (original index order, stride 64 bytes, pos
+ * at offset 0: the vertex attribute a renderer binds, Main.cpp:533-535). */
+int sph_device_buffer(sph_ctx* ctx, sph_particle_t** d_aos, size_t* n);
+
+/* The context's stream (a hipStream_t). */
+void* sph_stream(sph_ctx* ctx);
+int sph_synchronize(sph_ctx* ctx);
+
+/* Diagnostics: grid geometry of a scene; the neighbour grid built by the last
+ * rho_pres/step (cell_start has total_cells+1 entries over all scenes,
+ * perm[slot] = AoS index of the particle at cell-sorted slot). */
+int sph_grid_info(const sph_ctx* ctx, int scene, sph_grid_info_t* out);
+size_t sph_total_cells(const sph_ctx* ctx);
+int sph_download_grid(sph_ctx* ctx, uint32_t* cell_start, size_t ncell_entries, uint32_t* perm, size_t n);
+
+/* Reductions over scene `scene` of the AoS state (waits). */
+int sph_metrics(sph_ctx* ctx, int scene, sph_metrics_t* out);
+
+/* Kernel timing: while enabled, passes run without graphs and every kernel is
+ * bracketed by HIP events on the context's stream.  sph_kernel_times returns
+ * the accumulated milliseconds and launch counts per kernel name. */
+int sph_set_timing(sph_ctx* ctx, int enabled);
+int sph_kernel_times(sph_ctx* ctx, int max_entries, char (*names)[32], double* total_ms, int64_t* launches);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NPR_SPH_AMD_SPH_H */

@@ -1,0 +1,833 @@
+// sph_api.hip - C ABI of the solver (include/sph.h): buffer ownership, the
+// three-pass surface, graph-captured steps, diagnostics and kernel timing.
+//
+// Host counterpart in the reference: NPR-SPH/Main.cpp (init_particles
+// :510-539, sendUniforms :254-282, display :291-304, reload_shader
+// :428-451) and InitShader.cpp:47-112 for the error convention.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "sph.h"
+#include "sph_internal.h"
+
+using namespace nprsph;
+
+namespace {
+
+thread_local std::string g_create_error;
+
+constexpr double kPi = (double)3.141592741f;  // rho_pres_comp.glsl:8
+constexpr int kUnroll = 16;                    // steps per captured graph
+
+struct GridGeom {
+    float lo[3];
+    float inv_cs;
+    double cs;
+    int32_t dims[3];
+    uint32_t ncells;
+};
+
+// Cell edge h*(1+2^-10): with float32 rounding of the hash, |xi-xj| < h still
+// lands in adjacent cells.  Grown by 2^(1/4) until each axis has <= 2048
+// cells and the grid has <= max(65536, 8n) cells.
+bool grid_geometry(float h, const sph_boundary_t& b, uint64_t n, GridGeom& g)
+{
+    if (!(h > 0.0f) || !std::isfinite(h)) return false;
+    for (int a = 0; a < 3; ++a)
+        if (!std::isfinite(b.lower[a]) || !std::isfinite(b.upper[a])) return false;
+    double cs = (double)h * (1.0 + 1.0 / 1024.0);
+    const uint64_t cap = std::max<uint64_t>(65536u, 8u * n);
+    for (;;) {
+        uint64_t total = 1;
+        bool ok = true;
+        for (int a = 0; a < 3; ++a) {
+            const double ext = (double)b.upper[a] - (double)b.lower[a];
+            double d = std::ceil(ext / cs);
+            if (!(d >= 1.0)) d = 1.0;
+            if (d > 2048.0) {
+                ok = false;
+                d = 2048.0;
+            }
+            g.dims[a] = (int32_t)d;
+            total *= (uint64_t)g.dims[a];
+        }
+        if (ok && total <= cap) {
+            g.ncells = (uint32_t)total;
+            break;
+        }
+        cs *= 1.189207115002721;  // 2^(1/4)
+    }
+    g.cs = cs;
+    g.inv_cs = (float)(1.0 / cs);
+    for (int a = 0; a < 3; ++a) g.lo[a] = b.lower[a];
+    return true;
+}
+
+bool finite_all(const float* v, int n)
+{
+    for (int i = 0; i < n; ++i)
+        if (!std::isfinite(v[i])) return false;
+    return true;
+}
+
+struct PendingTimer {
+    const char* name;
+    hipEvent_t e0, e1;
+};
+
+}  // namespace
+
+struct sph_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    int nscenes = 0;
+    uint32_t n = 0;
+    std::vector<uint32_t> first, count;
+    std::vector<int> tile_begin, tile_end;
+    std::vector<sph_constants_t> C;
+    std::vector<sph_boundary_t> B;
+    std::vector<sph_fixed_t> F;
+    std::vector<GridGeom> geom;
+    std::vector<SceneDev> sd;
+    std::vector<Tile> tiles_h;
+    uint32_t total_cells = 0;
+    uint32_t cell_cap = 0;  // capacity of count/start (start has cell_cap+1)
+    int bsum_cap = 0;
+
+    DeviceState d{};
+    SceneDev* d_scenes = nullptr;
+    Tile* d_tiles = nullptr;
+
+    bool b_valid = false;    // B pos/vel == AoS pos/vel through perm
+    bool a_valid = false;    // A pos/vel == AoS through perm and the grid matches A
+    bool rho_valid = false;  // rho/D2 (A order) == AoS extras through perm
+
+    hipGraphExec_t graph[2] = {nullptr, nullptr};  // 1 step, kUnroll steps
+
+    bool timing = false;
+    std::vector<PendingTimer> pending;
+    std::vector<hipEvent_t> event_pool;
+    std::map<std::string, std::pair<double, int64_t>> times;
+
+    std::string err;
+};
+
+namespace {
+
+int fail(sph_ctx* c, int code, const std::string& msg)
+{
+    c->err = msg;
+    return code;
+}
+
+int hip_fail(sph_ctx* c, hipError_t e, const char* what)
+{
+    (void)hipGetLastError();
+    return fail(c, SPH_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define SPH_HIP(ctx, call)                                  \
+    do {                                                    \
+        hipError_t e_ = (call);                             \
+        if (e_ != hipSuccess) return hip_fail(ctx, e_, #call); \
+    } while (0)
+
+template <class T>
+hipError_t dalloc(T** p, size_t count)
+{
+    *p = nullptr;
+    if (count == 0) count = 1;
+    return hipMalloc((void**)p, count * sizeof(T));
+}
+
+SceneDev make_scene(const sph_constants_t& c, const sph_boundary_t& b, const sph_fixed_t& f, const GridGeom& g,
+                    uint32_t cell_base, uint32_t p_begin, uint32_t p_end)
+{
+    SceneDev s{};
+    for (int a = 0; a < 3; ++a) {
+        s.lo[a] = g.lo[a];
+        s.dims[a] = g.dims[a];
+        s.grav[a] = f.gravity[a];
+        s.lower[a] = b.lower[a];
+        s.upper[a] = b.upper[a];
+    }
+    s.inv_cs = g.inv_cs;
+    s.cell_base = cell_base;
+    s.ncells = g.ncells;
+    s.p_begin = p_begin;
+    s.p_end = p_end;
+    const float h = c.smoothing_coeff * f.particle_radius;  // rho_pres_comp.glsl:40, float32
+    const double hd = (double)h;
+    s.h = h;
+    s.h2 = h * h;
+    s.poly6_m = (float)((double)c.mass * 315.0 / (64.0 * kPi * std::pow(hd, 9)));
+    const double k45 = 45.0 / (kPi * std::pow(hd, 6));
+    s.pres_m = (float)((double)c.mass * k45 * 0.5);
+    s.visc_m = (float)((double)c.mass * k45 * (double)c.visc);
+    s.rho0 = c.resting_rho;
+    s.gas_const = f.gas_const;
+    s.dt = f.dt;
+    s.damping = f.damping;
+    s.mass = c.mass;
+    return s;
+}
+
+int validate_params(sph_ctx* c, const sph_constants_t& k, const sph_boundary_t& b, const sph_fixed_t& f)
+{
+    const float kc[4] = {k.mass, k.smoothing_coeff, k.visc, k.resting_rho};
+    const float kf[7] = {f.particle_radius, f.gas_const, f.gravity[0], f.gravity[1], f.gravity[2], f.damping, f.dt};
+    if (!finite_all(kc, 4) || !finite_all(kf, 7) || !finite_all(b.upper, 3) || !finite_all(b.lower, 3))
+        return fail(c, SPH_ERR_INVALID, "non-finite constant, boundary or fixed parameter");
+    const float h = k.smoothing_coeff * f.particle_radius;
+    if (!(h > 0.0f) || !std::isfinite(h)) return fail(c, SPH_ERR_INVALID, "smoothing length must be finite and > 0");
+    return SPH_OK;
+}
+
+void destroy_graphs(sph_ctx* c)
+{
+    for (auto& g : c->graph) {
+        if (g) (void)hipGraphExecDestroy(g);
+        g = nullptr;
+    }
+}
+
+// (Re)compute every scene's grid and parameter block; grow the cell buffers
+// when needed and upload the parameter blocks.
+int apply_params(sph_ctx* c, bool geometry_may_change)
+{
+    uint32_t cells = 0;
+    std::vector<GridGeom> geom(c->nscenes);
+    for (int s = 0; s < c->nscenes; ++s) {
+        if (!grid_geometry(c->C[s].smoothing_coeff * c->F[s].particle_radius, c->B[s], c->count[s], geom[s]))
+            return fail(c, SPH_ERR_INVALID, "invalid grid geometry");
+        const uint64_t next = (uint64_t)cells + geom[s].ncells + 1u;  // + NaN cell
+        if (next > 0x7fffffffull) return fail(c, SPH_ERR_INVALID, "too many grid cells");
+        cells = (uint32_t)next;
+    }
+    bool changed = cells != c->total_cells;
+    for (int s = 0; s < c->nscenes && !changed; ++s)
+        changed = std::memcmp(&geom[s], &c->geom[s], sizeof(GridGeom)) != 0;
+    (void)geometry_may_change;
+
+    if (cells > c->cell_cap) {
+        SPH_HIP(c, hipStreamSynchronize(c->stream));
+        if (c->d.count) SPH_HIP(c, hipFree(c->d.count));
+        if (c->d.start) SPH_HIP(c, hipFree(c->d.start));
+        c->d.count = c->d.start = nullptr;
+        const uint32_t cap = cells + cells / 4u;
+        if (dalloc(&c->d.count, cap) != hipSuccess || dalloc(&c->d.start, (size_t)cap + 1u) != hipSuccess) {
+            (void)hipGetLastError();
+            c->cell_cap = 0;
+            return fail(c, SPH_ERR_NOMEM, "cannot allocate grid cells");
+        }
+        c->cell_cap = cap;
+        hipError_t e = launch_fill_u32(c->stream, c->d.count, cap, 0u);
+        if (e != hipSuccess) return hip_fail(c, e, "clear cell counts");
+    }
+    const int nb = scan_blocks(cells);
+    if (nb > c->bsum_cap) {
+        SPH_HIP(c, hipStreamSynchronize(c->stream));
+        if (c->d.bsum) SPH_HIP(c, hipFree(c->d.bsum));
+        c->d.bsum = nullptr;
+        if (dalloc(&c->d.bsum, (size_t)nb) != hipSuccess) {
+            (void)hipGetLastError();
+            c->bsum_cap = 0;
+            return fail(c, SPH_ERR_NOMEM, "cannot allocate scan scratch");
+        }
+        c->bsum_cap = nb;
+    }
+
+    uint32_t base = 0;
+    for (int s = 0; s < c->nscenes; ++s) {
+        c->sd[s] = make_scene(c->C[s], c->B[s], c->F[s], geom[s], base, c->first[s], c->first[s] + c->count[s]);
+        base += geom[s].ncells + 1u;
+        hipError_t e = launch_set_scene(c->stream, c->d_scenes + s, c->sd[s]);
+        if (e != hipSuccess) return hip_fail(c, e, "upload scene parameters");
+    }
+    c->geom = geom;
+    c->total_cells = cells;
+    c->d.total_cells = cells;
+    if (changed) {
+        destroy_graphs(c);
+        c->a_valid = false;
+    }
+    return SPH_OK;
+}
+
+hipEvent_t take_event(sph_ctx* c)
+{
+    if (!c->event_pool.empty()) {
+        hipEvent_t e = c->event_pool.back();
+        c->event_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) {
+        (void)hipGetLastError();
+        return nullptr;
+    }
+    return e;
+}
+
+// Enqueue one kernel through its launcher; in timing mode bracket it with
+// events on the context's stream.
+template <class F>
+int run(sph_ctx* c, const char* name, F&& launch)
+{
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (c->timing) {
+        e0 = take_event(c);
+        e1 = take_event(c);
+        if (e0) (void)hipEventRecord(e0, c->stream);
+    }
+    const hipError_t e = launch();
+    if (c->timing && e0 && e1) {
+        (void)hipEventRecord(e1, c->stream);
+        c->pending.push_back({name, e0, e1});
+    }
+    if (e != hipSuccess) return hip_fail(c, e, name);
+    return SPH_OK;
+}
+
+#define SPH_RUN(ctx, name, expr)                           \
+    do {                                                   \
+        int rc_ = run(ctx, name, [&]() { return (expr); }); \
+        if (rc_ != SPH_OK) return rc_;                     \
+    } while (0)
+
+int enqueue_build(sph_ctx* c)
+{
+    const DeviceState& d = c->d;
+    SPH_RUN(c, "hash_count", launch_hash_count(c->stream, d));
+    SPH_RUN(c, "scan_reduce", launch_scan_reduce(c->stream, d));
+    SPH_RUN(c, "scan_top", launch_scan_top(c->stream, d));
+    SPH_RUN(c, "scan_down", launch_scan_down(c->stream, d));
+    SPH_RUN(c, "scatter", launch_scatter(c->stream, d));
+    SPH_RUN(c, "rank_reorder", launch_rank_reorder(c->stream, d));
+    return SPH_OK;
+}
+
+int enqueue_step(sph_ctx* c)
+{
+    int rc = enqueue_build(c);
+    if (rc != SPH_OK) return rc;
+    SPH_RUN(c, "density", launch_density(c->stream, c->d, false));
+    SPH_RUN(c, "force_integrate", launch_force(c->stream, c->d, true));
+    return SPH_OK;
+}
+
+int ensure_b(sph_ctx* c)
+{
+    if (!c->b_valid) {
+        SPH_RUN(c, "refresh", launch_refresh(c->stream, c->d, true, false));
+        c->b_valid = true;
+    }
+    return SPH_OK;
+}
+
+int capture_graph(sph_ctx* c, int which, int steps)
+{
+    hipGraph_t g = nullptr;
+    SPH_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    int rc = SPH_OK;
+    for (int k = 0; k < steps && rc == SPH_OK; ++k) rc = enqueue_step(c);
+    hipError_t e = hipStreamEndCapture(c->stream, &g);
+    if (rc != SPH_OK) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (e != hipSuccess) return hip_fail(c, e, "hipStreamEndCapture");
+    e = hipGraphInstantiate(&c->graph[which], g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) {
+        c->graph[which] = nullptr;
+        return hip_fail(c, e, "hipGraphInstantiate");
+    }
+    return SPH_OK;
+}
+
+void invalidate_state(sph_ctx* c)
+{
+    c->a_valid = c->b_valid = c->rho_valid = false;
+}
+
+int check_scene(sph_ctx* c, int scene)
+{
+    if (scene < -1 || scene >= c->nscenes) return fail(c, SPH_ERR_INVALID, "scene index out of range");
+    return SPH_OK;
+}
+
+int set_device(sph_ctx* c)
+{
+    SPH_HIP(c, hipSetDevice(c->device));
+    return SPH_OK;
+}
+
+void free_all(sph_ctx* c)
+{
+    destroy_graphs(c);
+    for (auto& p : c->pending) {
+        (void)hipEventDestroy(p.e0);
+        (void)hipEventDestroy(p.e1);
+    }
+    for (auto e : c->event_pool) (void)hipEventDestroy(e);
+    void* bufs[] = {c->d.aos, c->d.PA, c->d.VA, c->d.PB, c->d.VB, c->d.rho, c->d.D2, c->d.perm, c->d.key,
+                    c->d.slot, c->d.torig, c->d.tsrc, c->d.tkey, c->d.count, c->d.start, c->d.bsum, c->d.partials,
+                    c->d_scenes, c->d_tiles};
+    for (void* p : bufs)
+        if (p) (void)hipFree(p);
+    if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
+}
+
+}  // namespace
+
+extern "C" {
+
+void sph_defaults(sph_constants_t* c, sph_boundary_t* b, sph_fixed_t* f)
+{
+    if (c) *c = sph_constants_t{0.02f, 4.0f, 3000.0f, 1000.0f};  // Main.cpp:110-116
+    if (b) *b = sph_boundary_t{{0.5f, 1.0f, 0.5f, 1.0f}, {-0.1f, -0.35f, -0.1f, 1.0f}};  // Main.cpp:118-122
+    if (f) {
+        f->particle_radius = 0.005f;  // rho_pres_comp.glsl:5
+        f->gas_const = 2000.0f;       // rho_pres_comp.glsl:33
+        f->gravity[0] = 0.0f;         // force_comp.glsl:33
+        f->gravity[1] = -9806.65f;
+        f->gravity[2] = 0.0f;
+        f->damping = 0.3f;              // integrate_comp.glsl:8
+        f->dt = 1.0f / 10000.0f;        // integrate_comp.glsl:33 at NUM_PARTICLES = 10000
+    }
+}
+
+const char* sph_create_error(void) { return g_create_error.c_str(); }
+
+int sph_create_batch(sph_ctx** out, int nscenes, const size_t* counts, const sph_constants_t* cs,
+                     const sph_boundary_t* bs, const sph_fixed_t* fs, void* stream)
+{
+    g_create_error.clear();
+    if (!out) {
+        g_create_error = "out is null";
+        return SPH_ERR_INVALID;
+    }
+    *out = nullptr;
+    if (nscenes < 1 || !counts) {
+        g_create_error = "need at least one scene";
+        return SPH_ERR_INVALID;
+    }
+    uint64_t total = 0;
+    for (int s = 0; s < nscenes; ++s) {
+        if (counts[s] < 1) {
+            g_create_error = "every scene needs at least one particle";
+            return SPH_ERR_INVALID;
+        }
+        total += counts[s];
+    }
+    if (total > 0x7fffffffull) {
+        g_create_error = "too many particles (limit 2^31-1)";
+        return SPH_ERR_INVALID;
+    }
+    sph_ctx* c = new sph_ctx();
+    auto bail = [&](int code) {
+        g_create_error = c->err;
+        free_all(c);
+        delete c;
+        return code;
+    };
+    if (hipGetDevice(&c->device) != hipSuccess) {
+        c->err = "no HIP device";
+        (void)hipGetLastError();
+        return bail(SPH_ERR_HIP);
+    }
+    if (stream) {
+        c->stream = (hipStream_t)stream;
+    } else {
+        if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+            c->err = "hipStreamCreate failed";
+            (void)hipGetLastError();
+            return bail(SPH_ERR_HIP);
+        }
+        c->own_stream = true;
+    }
+    c->nscenes = nscenes;
+    c->n = (uint32_t)total;
+    c->first.resize(nscenes);
+    c->count.resize(nscenes);
+    c->C.resize(nscenes);
+    c->B.resize(nscenes);
+    c->F.resize(nscenes);
+    c->geom.resize(nscenes);
+    c->sd.resize(nscenes);
+    c->tile_begin.resize(nscenes);
+    c->tile_end.resize(nscenes);
+    uint32_t off = 0;
+    for (int s = 0; s < nscenes; ++s) {
+        c->first[s] = off;
+        c->count[s] = (uint32_t)counts[s];
+        sph_defaults(&c->C[s], &c->B[s], &c->F[s]);
+        if (cs) c->C[s] = cs[s];
+        if (bs) c->B[s] = bs[s];
+        if (fs) c->F[s] = fs[s];
+        int rc = validate_params(c, c->C[s], c->B[s], c->F[s]);
+        if (rc != SPH_OK) return bail(rc);
+        c->tile_begin[s] = (int)c->tiles_h.size();
+        for (uint32_t b = 0; b < c->count[s]; b += kTile)
+            c->tiles_h.push_back(Tile{(uint32_t)s, off + b, off + std::min<uint32_t>(b + kTile, c->count[s]), 0u});
+        c->tile_end[s] = (int)c->tiles_h.size();
+        off += c->count[s];
+    }
+    const uint32_t n = c->n;
+    DeviceState& d = c->d;
+    d.n = n;
+    d.ntiles = (int)c->tiles_h.size();
+    bool ok = dalloc(&d.aos, n) == hipSuccess && dalloc(&d.PA, n) == hipSuccess && dalloc(&d.VA, n) == hipSuccess &&
+              dalloc(&d.PB, n) == hipSuccess && dalloc(&d.VB, n) == hipSuccess && dalloc(&d.rho, n) == hipSuccess &&
+              dalloc(&d.D2, n) == hipSuccess && dalloc(&d.perm, n) == hipSuccess && dalloc(&d.key, n) == hipSuccess &&
+              dalloc(&d.slot, n) == hipSuccess && dalloc(&d.torig, n) == hipSuccess &&
+              dalloc(&d.tsrc, n) == hipSuccess && dalloc(&d.tkey, n) == hipSuccess &&
+              dalloc(&d.partials, (size_t)d.ntiles * 8u) == hipSuccess &&
+              dalloc(&c->d_scenes, (size_t)nscenes) == hipSuccess && dalloc(&c->d_tiles, (size_t)d.ntiles) == hipSuccess;
+    if (!ok) {
+        (void)hipGetLastError();
+        c->err = "device allocation failed";
+        return bail(SPH_ERR_NOMEM);
+    }
+    d.scenes = c->d_scenes;
+    d.tiles = c->d_tiles;
+    if (hipMemcpyAsync(c->d_tiles, c->tiles_h.data(), sizeof(Tile) * c->tiles_h.size(), hipMemcpyHostToDevice,
+                       c->stream) != hipSuccess) {
+        (void)hipGetLastError();
+        c->err = "tile upload failed";
+        return bail(SPH_ERR_HIP);
+    }
+    int rc = apply_params(c, true);
+    if (rc != SPH_OK) return bail(rc);
+    if (launch_fill_aos_defaults(c->stream, d.aos, n) != hipSuccess || launch_identity(c->stream, d.perm, n) != hipSuccess ||
+        hipStreamSynchronize(c->stream) != hipSuccess) {
+        (void)hipGetLastError();
+        c->err = "initialisation kernels failed";
+        return bail(SPH_ERR_HIP);
+    }
+    *out = c;
+    return SPH_OK;
+}
+
+int sph_create(sph_ctx** out, size_t n, const sph_constants_t* c, const sph_boundary_t* b, const sph_fixed_t* f,
+               void* stream)
+{
+    return sph_create_batch(out, 1, &n, c, b, f, stream);
+}
+
+void sph_destroy(sph_ctx* c)
+{
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    free_all(c);
+    delete c;
+}
+
+const char* sph_last_error(const sph_ctx* c) { return c ? c->err.c_str() : "null context"; }
+size_t sph_size(const sph_ctx* c) { return c ? c->n : 0; }
+int sph_num_scenes(const sph_ctx* c) { return c ? c->nscenes : 0; }
+void* sph_stream(sph_ctx* c) { return c ? (void*)c->stream : nullptr; }
+
+static int set_param_common(sph_ctx* c, int scene, const sph_constants_t* k, const sph_boundary_t* b,
+                            const sph_fixed_t* f)
+{
+    if (!c) return SPH_ERR_INVALID;
+    int rc = check_scene(c, scene);
+    if (rc != SPH_OK) return rc;
+    rc = set_device(c);
+    if (rc != SPH_OK) return rc;
+    const int s0 = scene < 0 ? 0 : scene, s1 = scene < 0 ? c->nscenes : scene + 1;
+    for (int s = s0; s < s1; ++s) {
+        rc = validate_params(c, k ? *k : c->C[s], b ? *b : c->B[s], f ? *f : c->F[s]);
+        if (rc != SPH_OK) return rc;
+    }
+    auto oldC = c->C;
+    auto oldB = c->B;
+    auto oldF = c->F;
+    for (int s = s0; s < s1; ++s) {
+        if (k) c->C[s] = *k;
+        if (b) c->B[s] = *b;
+        if (f) c->F[s] = *f;
+    }
+    rc = apply_params(c, true);
+    if (rc != SPH_OK) {
+        c->C = oldC;
+        c->B = oldB;
+        c->F = oldF;
+    }
+    return rc;
+}
+
+int sph_set_constants(sph_ctx* c, int scene, const sph_constants_t* k)
+{
+    if (!k) return c ? fail(c, SPH_ERR_INVALID, "constants is null") : SPH_ERR_INVALID;
+    return set_param_common(c, scene, k, nullptr, nullptr);
+}
+
+int sph_set_boundary(sph_ctx* c, int scene, const sph_boundary_t* b)
+{
+    if (!b) return c ? fail(c, SPH_ERR_INVALID, "boundary is null") : SPH_ERR_INVALID;
+    return set_param_common(c, scene, nullptr, b, nullptr);
+}
+
+int sph_set_fixed(sph_ctx* c, int scene, const sph_fixed_t* f)
+{
+    if (!f) return c ? fail(c, SPH_ERR_INVALID, "fixed is null") : SPH_ERR_INVALID;
+    return set_param_common(c, scene, nullptr, nullptr, f);
+}
+
+int sph_get_params(const sph_ctx* c, int scene, sph_constants_t* k, sph_boundary_t* b, sph_fixed_t* f)
+{
+    if (!c || scene < 0 || scene >= c->nscenes) return SPH_ERR_INVALID;
+    if (k) *k = c->C[scene];
+    if (b) *b = c->B[scene];
+    if (f) *f = c->F[scene];
+    return SPH_OK;
+}
+
+int sph_upload(sph_ctx* c, const sph_particle_t* host, size_t first, size_t n)
+{
+    if (!c) return SPH_ERR_INVALID;
+    if (!host && n) return fail(c, SPH_ERR_INVALID, "host pointer is null");
+    if (first > c->n || n > c->n - first) return fail(c, SPH_ERR_INVALID, "range exceeds particle count");
+    int rc = set_device(c);
+    if (rc != SPH_OK) return rc;
+    if (n) SPH_HIP(c, hipMemcpyAsync(c->d.aos + first, host, n * sizeof(sph_particle_t), hipMemcpyHostToDevice, c->stream));
+    invalidate_state(c);
+    return SPH_OK;
+}
+
+int sph_download(sph_ctx* c, sph_particle_t* host, size_t first, size_t n)
+{
+    if (!c) return SPH_ERR_INVALID;
+    if (!host && n) return fail(c, SPH_ERR_INVALID, "host pointer is null");
+    if (first > c->n || n > c->n - first) return fail(c, SPH_ERR_INVALID, "range exceeds particle count");
+    int rc = set_device(c);
+    if (rc != SPH_OK) return rc;
+    if (n) SPH_HIP(c, hipMemcpyAsync(host, c->d.aos + first, n * sizeof(sph_particle_t), hipMemcpyDeviceToHost, c->stream));
+    SPH_HIP(c, hipStreamSynchronize(c->stream));
+    return SPH_OK;
+}
+
+int sph_init_lattice(sph_ctx* c, int scene, const sph_lattice_t* spec)
+{
+    if (!c) return SPH_ERR_INVALID;
+    if (!spec) return fail(c, SPH_ERR_INVALID, "lattice spec is null");
+    if (scene < 0 || scene >= c->nscenes) return fail(c, SPH_ERR_INVALID, "scene index out of range");
+    if (spec->nx < 1 || spec->ny < 1 || spec->nz < 1) return fail(c, SPH_ERR_INVALID, "lattice dims must be >= 1");
+    const uint64_t sites = (uint64_t)spec->nx * (uint64_t)spec->ny * (uint64_t)spec->nz;
+    if (spec->count != c->count[scene] || (uint64_t)spec->count > sites)
+        return fail(c, SPH_ERR_INVALID, "lattice count must equal the scene's particle count and fit the lattice");
+    if (!finite_all(spec->origin, 3) || !std::isfinite(spec->spacing) || !std::isfinite(spec->jitter))
+        return fail(c, SPH_ERR_INVALID, "non-finite lattice parameter");
+    int rc = set_device(c);
+    if (rc != SPH_OK) return rc;
+    SPH_RUN(c, "lattice", launch_lattice(c->stream, c->d.aos, c->first[scene], *spec));
+    invalidate_state(c);
+    return SPH_OK;
+}
+
+// rho_pres_comp.glsl:35-58: rebuild the neighbour grid from the AoS positions,
+// then the density gather; writes extras.x = rho and extras.y = p.
+int sph_rho_pres(sph_ctx* c)
+{
+    if (!c) return SPH_ERR_INVALID;
+    int rc = set_device(c);
+    if (rc != SPH_OK) return rc;
+    if ((rc = ensure_b(c)) != SPH_OK) return rc;
+    if ((rc = enqueue_build(c)) != SPH_OK) return rc;
+    c->b_valid = false;
+    c->a_valid = true;
+    SPH_RUN(c, "density", launch_density(c->stream, c->d, true));
+    c->rho_valid = true;
+    return SPH_OK;
+}
+
+// force_comp.glsl:35-66: reads pos, vel and extras.xy; writes force.xyz.
+int sph_force(sph_ctx* c)
+{
+    if (!c) return SPH_ERR_INVALID;
+    int rc = set_device(c);
+    if (rc != SPH_OK) return rc;
+    if (!c->a_valid) {
+        if ((rc = ensure_b(c)) != SPH_OK) return rc;
+        if ((rc = enqueue_build(c)) != SPH_OK) return rc;
+        c->b_valid = false;
+        c->a_valid = true;
+        c->rho_valid = false;
+    }
+    if (!c->rho_valid) {
+        SPH_RUN(c, "refresh_extras", launch_refresh(c->stream, c->d, false, true));
+        c->rho_valid = true;
+    }
+    SPH_RUN(c, "force", launch_force(c->stream, c->d, false));
+    return SPH_OK;
+}
+
+// integrate_comp.glsl:35-81 on the AoS buffer.
+int sph_integrate(sph_ctx* c)
+{
+    if (!c) return SPH_ERR_INVALID;
+    int rc = set_device(c);
+    if (rc != SPH_OK) return rc;
+    SPH_RUN(c, "integrate", launch_integrate_aos(c->stream, c->d));
+    invalidate_state(c);
+    return SPH_OK;
+}
+
+int sph_step(sph_ctx* c, int nsteps)
+{
+    if (!c) return SPH_ERR_INVALID;
+    if (nsteps < 0) return fail(c, SPH_ERR_INVALID, "nsteps must be >= 0");
+    if (nsteps == 0) return SPH_OK;
+    int rc = set_device(c);
+    if (rc != SPH_OK) return rc;
+    if ((rc = ensure_b(c)) != SPH_OK) return rc;
+    if (c->timing) {
+        for (int k = 0; k < nsteps; ++k)
+            if ((rc = enqueue_step(c)) != SPH_OK) return rc;
+    } else {
+        int big = nsteps / kUnroll, small = nsteps % kUnroll;
+        if (big && !c->graph[1] && (rc = capture_graph(c, 1, kUnroll)) != SPH_OK) return rc;
+        if (small && !c->graph[0] && (rc = capture_graph(c, 0, 1)) != SPH_OK) return rc;
+        for (int k = 0; k < big; ++k) SPH_HIP(c, hipGraphLaunch(c->graph[1], c->stream));
+        for (int k = 0; k < small; ++k) SPH_HIP(c, hipGraphLaunch(c->graph[0], c->stream));
+    }
+    c->b_valid = true;
+    c->a_valid = false;
+    c->rho_valid = false;
+    return SPH_OK;
+}
+
+int sph_device_buffer(sph_ctx* c, sph_particle_t** d_aos, size_t* n)
+{
+    if (!c) return SPH_ERR_INVALID;
+    if (d_aos) *d_aos = c->d.aos;
+    if (n) *n = c->n;
+    return SPH_OK;
+}
+
+int sph_synchronize(sph_ctx* c)
+{
+    if (!c) return SPH_ERR_INVALID;
+    int rc = set_device(c);
+    if (rc != SPH_OK) return rc;
+    SPH_HIP(c, hipStreamSynchronize(c->stream));
+    return SPH_OK;
+}
+
+int sph_grid_info(const sph_ctx* c, int scene, sph_grid_info_t* out)
+{
+    if (!c || !out || scene < 0 || scene >= c->nscenes) return SPH_ERR_INVALID;
+    const GridGeom& g = c->geom[scene];
+    for (int a = 0; a < 3; ++a) {
+        out->lo[a] = g.lo[a];
+        out->dims[a] = g.dims[a];
+    }
+    out->inv_cell = g.inv_cs;
+    out->cell = g.cs;
+    out->ncells = g.ncells;
+    out->cell_base = c->sd[scene].cell_base;
+    out->first = c->first[scene];
+    out->count = c->count[scene];
+    return SPH_OK;
+}
+
+size_t sph_total_cells(const sph_ctx* c) { return c ? c->total_cells : 0; }
+
+int sph_download_grid(sph_ctx* c, uint32_t* cell_start, size_t ncell_entries, uint32_t* perm, size_t n)
+{
+    if (!c) return SPH_ERR_INVALID;
+    int rc = set_device(c);
+    if (rc != SPH_OK) return rc;
+    if (cell_start) {
+        if (ncell_entries != (size_t)c->total_cells + 1u) return fail(c, SPH_ERR_INVALID, "cell_start needs total_cells+1 entries");
+        SPH_HIP(c, hipMemcpyAsync(cell_start, c->d.start, ncell_entries * 4u, hipMemcpyDeviceToHost, c->stream));
+    }
+    if (perm) {
+        if (n != c->n) return fail(c, SPH_ERR_INVALID, "perm needs one entry per particle");
+        SPH_HIP(c, hipMemcpyAsync(perm, c->d.perm, n * 4u, hipMemcpyDeviceToHost, c->stream));
+    }
+    SPH_HIP(c, hipStreamSynchronize(c->stream));
+    return SPH_OK;
+}
+
+int sph_metrics(sph_ctx* c, int scene, sph_metrics_t* out)
+{
+    if (!c) return SPH_ERR_INVALID;
+    if (!out || scene < 0 || scene >= c->nscenes) return fail(c, SPH_ERR_INVALID, "bad metrics arguments");
+    int rc = set_device(c);
+    if (rc != SPH_OK) return rc;
+    const int t0 = c->tile_begin[scene], nt = c->tile_end[scene] - t0;
+    SPH_RUN(c, "metrics", launch_metrics(c->stream, c->d, t0, nt));
+    std::vector<double> part((size_t)nt * 8u);
+    SPH_HIP(c, hipMemcpyAsync(part.data(), c->d.partials + (size_t)t0 * 8u, part.size() * sizeof(double),
+                              hipMemcpyDeviceToHost, c->stream));
+    SPH_HIP(c, hipStreamSynchronize(c->stream));
+    double acc[7] = {0, 0, 0, 0, 0, 0, 0};
+    for (int t = 0; t < nt; ++t)
+        for (int k = 0; k < 7; ++k) acc[k] = (k == 5) ? std::max(acc[k], part[(size_t)t * 8 + k]) : acc[k] + part[(size_t)t * 8 + k];
+    const double cnt = (double)c->count[scene];
+    out->mean_rho = acc[0] / cnt;
+    out->kinetic = acc[1];
+    out->com[0] = acc[2] / cnt;
+    out->com[1] = acc[3] / cnt;
+    out->com[2] = acc[4] / cnt;
+    out->max_speed = acc[5];
+    out->nonfinite = (uint64_t)acc[6];
+    return SPH_OK;
+}
+
+int sph_set_timing(sph_ctx* c, int enabled)
+{
+    if (!c) return SPH_ERR_INVALID;
+    c->timing = enabled != 0;
+    return SPH_OK;
+}
+
+int sph_kernel_times(sph_ctx* c, int max_entries, char (*names)[32], double* total_ms, int64_t* launches)
+{
+    if (!c) return SPH_ERR_INVALID;
+    int rc = set_device(c);
+    if (rc != SPH_OK) return rc;
+    SPH_HIP(c, hipStreamSynchronize(c->stream));
+    for (auto& p : c->pending) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, p.e0, p.e1) == hipSuccess) {
+            auto& slot = c->times[p.name];
+            slot.first += ms;
+            slot.second += 1;
+        } else {
+            (void)hipGetLastError();
+        }
+        c->event_pool.push_back(p.e0);
+        c->event_pool.push_back(p.e1);
+    }
+    c->pending.clear();
+    int k = 0;
+    for (auto& kv : c->times) {
+        if (k >= max_entries) break;
+        if (names) {
+            std::snprintf(names[k], 32, "%s", kv.first.c_str());
+        }
+        if (total_ms) total_ms[k] = kv.second.first;
+        if (launches) launches[k] = kv.second.second;
+        ++k;
+    }
+    if (max_entries <= 0) return (int)c->times.size();
+    if (!names && !total_ms && !launches) return (int)c->times.size();
+    c->times.clear();
+    return k;
+}
+
+}  // extern "C"

@@ -1,0 +1,602 @@
+// sph_kernels.hip - CDNA4 (gfx950) kernels of the SPH solver.
+//
+// One simulation step over cell-sorted state:
+//   k_hash_count   cell key per particle + wave-aggregated per-cell counts
+//   k_scan_*       exclusive scan of the counts -> first slot of every cell
+//   k_scatter      arrival-order placement into cell ranges
+//   k_rank_reorder stable order inside each cell (by AoS index) + gather of
+//                  pos/vel into the sorted arrays the gathers read
+//   k_density      rho_pres_comp.glsl:35-58 as a 27-cell gather
+//   k_force        force_comp.glsl:35-66 as a 27-cell gather, optionally fused
+//                  with integrate_comp.glsl:35-81 and the AoS write-back
+//   k_integrate_aos integrate_comp.glsl:35-81 as a streaming pass over the AoS
+// Scenes are independent; every launch covers all of them through the tile
+// table (a tile is kTile consecutive particles of one scene).
+#include "sph_internal.h"
+
+namespace nprsph {
+
+namespace {
+
+constexpr int kWave = 64;
+
+__device__ __forceinline__ int lane_id() { return (int)__lane_id(); }
+
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v)
+{
+    const int lane = lane_id();
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+        const uint32_t t = __shfl_up(v, (unsigned)d);
+        if (lane >= d) v += t;
+    }
+    return v;
+}
+
+__device__ __forceinline__ uint32_t wave_sum(uint32_t v)
+{
+#pragma unroll
+    for (int d = kWave / 2; d > 0; d >>= 1) v += __shfl_xor(v, d);
+    return v;
+}
+
+// Exclusive scan across a workgroup of NT threads; returns the workgroup total
+// through `total`.
+template <int NT>
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t& total)
+{
+    constexpr int NW = NT / kWave;
+    __shared__ uint32_t wtot[NW];
+    const int lane = lane_id();
+    const int w = threadIdx.x / kWave;
+    const uint32_t inc = wave_incl_scan(v);
+    if (lane == kWave - 1) wtot[w] = inc;
+    __syncthreads();
+    uint32_t before = 0, all = 0;
+#pragma unroll
+    for (int k = 0; k < NW; ++k) {
+        const uint32_t t = wtot[k];
+        before += (k < w) ? t : 0u;
+        all += t;
+    }
+    __syncthreads();
+    total = all;
+    return before + inc - v;
+}
+
+template <int NT>
+__device__ __forceinline__ uint32_t block_sum(uint32_t v)
+{
+    constexpr int NW = NT / kWave;
+    __shared__ uint32_t wtot[NW];
+    v = wave_sum(v);
+    if (lane_id() == 0) wtot[threadIdx.x / kWave] = v;
+    __syncthreads();
+    uint32_t all = 0;
+#pragma unroll
+    for (int k = 0; k < NW; ++k) all += wtot[k];
+    __syncthreads();
+    return all;
+}
+
+// ------------------------------------------------------------ utilities ---
+
+__global__ void k_set_scene(SceneDev* dst, SceneDev value) { *dst = value; }
+
+__global__ __launch_bounds__(256) void k_fill_aos_defaults(sph_particle_t* __restrict__ aos, uint32_t n)
+{
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= n) return;
+    float4* r = reinterpret_cast<float4*>(aos + i);
+    r[0] = make_float4(0.f, 0.f, 0.f, 1.f);
+    r[1] = make_float4(0.f, 0.f, 0.f, 0.f);
+    r[2] = make_float4(0.f, 0.f, 0.f, 0.f);
+    r[3] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+__global__ __launch_bounds__(256) void k_fill_u32(uint32_t* __restrict__ p, uint32_t n, uint32_t value)
+{
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i < n) p[i] = value;
+}
+
+__global__ __launch_bounds__(256) void k_identity(uint32_t* __restrict__ p, uint32_t n)
+{
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i < n) p[i] = i;
+}
+
+// Seeded jittered lattice (make_grid/init_particles, Main.cpp:488-539, with
+// jitter).  Same float32 operation order as the CPU generator.
+__global__ __launch_bounds__(256) void k_lattice(sph_particle_t* __restrict__ aos, uint32_t first, sph_lattice_t spec)
+{
+#pragma clang fp contract(off)
+    const uint32_t k = blockIdx.x * 256u + threadIdx.x;
+    if (k >= spec.count) return;
+    const uint32_t nz = (uint32_t)spec.nz, ny = (uint32_t)spec.ny;
+    const uint32_t idx[3] = {k / (nz * ny), (k / nz) % ny, k % nz};
+    float p[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const float base = spec.origin[a] + (float)idx[a] * spec.spacing;
+        p[a] = base + lattice_jitter(spec.seed, 3ull * (uint64_t)k + (uint64_t)a, spec.jitter);
+    }
+    float4* r = reinterpret_cast<float4*>(aos + first + k);
+    r[0] = make_float4(p[0], p[1], p[2], 1.f);
+    r[1] = make_float4(0.f, 0.f, 0.f, 0.f);
+    r[2] = make_float4(0.f, 0.f, 0.f, 0.f);
+    r[3] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+// Copy AoS state into slot order B (pos/vel) and/or A (rho, p) through perm.
+__global__ __launch_bounds__(kTile) void k_refresh(const Tile* __restrict__ tiles, const sph_particle_t* __restrict__ aos,
+                                                   const uint32_t* __restrict__ perm, float4* __restrict__ P,
+                                                   float4* __restrict__ V, float* __restrict__ rho,
+                                                   float2* __restrict__ D2, int pos_vel, int extras)
+{
+    const Tile t = tiles[blockIdx.x];
+    const uint32_t s = t.begin + threadIdx.x;
+    if (s >= t.end) return;
+    const float4* r = reinterpret_cast<const float4*>(aos + perm[s]);
+    if (pos_vel) {
+        const float4 p = r[0], v = r[1];
+        P[s] = make_float4(p.x, p.y, p.z, 0.f);
+        V[s] = make_float4(v.x, v.y, v.z, 0.f);
+    }
+    if (extras) {
+        const float4 e = r[3];
+        rho[s] = e.x;
+        D2[s] = make_float2(e.y, 1.0f / e.x);
+    }
+}
+
+// ------------------------------------------------------- neighbour grid ---
+
+// Cell key of every slot of B, and its arrival rank inside the cell.  Lanes of
+// a wave that hold a run of equal keys (cell-sorted input makes runs the
+// common case) share one atomic.
+__global__ __launch_bounds__(kTile) void k_hash_count(const Tile* __restrict__ tiles, const SceneDev* __restrict__ scenes,
+                                                      const float4* __restrict__ P, uint32_t* __restrict__ key,
+                                                      uint32_t* __restrict__ slot, uint32_t* __restrict__ count)
+{
+    const Tile t = tiles[blockIdx.x];
+    const SceneDev& sc = scenes[t.scene];
+    const uint32_t s = t.begin + threadIdx.x;
+    const bool active = s < t.end;
+    uint32_t k = 0xffffffffu;
+    if (active) {
+        const float4 p = P[s];
+        k = global_key(sc, p.x, p.y, p.z);
+        key[s] = k;
+    }
+    const int lane = lane_id();
+    const uint32_t prev = __shfl_up(k, 1u);
+    const bool head = !active || lane == 0 || prev != k;
+    const uint64_t heads = __ballot(head);
+    const uint64_t upto = (lane == 63) ? ~0ull : ((2ull << lane) - 1ull);  // bits 0..lane
+    const int my_head = 63 - __clzll((long long)(heads & upto));
+    const uint64_t after = heads & ~upto;
+    const int next = after ? (__ffsll((long long)after) - 1) : kWave;
+    uint32_t base = 0;
+    if (head && active) base = atomicAdd(&count[k], (uint32_t)(next - lane));
+    base = __shfl(base, my_head);
+    if (active) slot[s] = base + (uint32_t)(lane - my_head);
+}
+
+__global__ __launch_bounds__(kScanThreads) void k_scan_reduce(const uint32_t* __restrict__ count, uint32_t n,
+                                                              uint32_t* __restrict__ bsum)
+{
+    const uint32_t base = blockIdx.x * (uint32_t)kScanBlock;
+    uint32_t sum = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        const uint32_t i = base + (uint32_t)(k * kScanThreads) + threadIdx.x;
+        if (i < n) sum += count[i];
+    }
+    sum = block_sum<kScanThreads>(sum);
+    if (threadIdx.x == 0) bsum[blockIdx.x] = sum;
+}
+
+__global__ __launch_bounds__(1024) void k_scan_top(uint32_t* __restrict__ bsum, uint32_t nb)
+{
+    uint32_t carry = 0;
+    for (uint32_t base = 0; base < nb; base += 1024u) {
+        const uint32_t i = base + threadIdx.x;
+        const uint32_t v = i < nb ? bsum[i] : 0u;
+        uint32_t total;
+        const uint32_t ex = block_excl_scan<1024>(v, total);
+        if (i < nb) bsum[i] = carry + ex;
+        carry += total;
+    }
+}
+
+// start[c] = exclusive prefix of count; counts are cleared for the next build
+// and start[n] receives the total.
+__global__ __launch_bounds__(kScanThreads) void k_scan_down(uint32_t* __restrict__ count, uint32_t n,
+                                                            const uint32_t* __restrict__ bsum, uint32_t* __restrict__ start)
+{
+    const uint32_t base = blockIdx.x * (uint32_t)kScanBlock + threadIdx.x * (uint32_t)kScanItems;
+    uint32_t v[kScanItems];
+    uint32_t sum = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        const uint32_t i = base + (uint32_t)k;
+        v[k] = i < n ? count[i] : 0u;
+        sum += v[k];
+    }
+    uint32_t total;
+    uint32_t run = bsum[blockIdx.x] + block_excl_scan<kScanThreads>(sum, total);
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        const uint32_t i = base + (uint32_t)k;
+        if (i < n) {
+            start[i] = run;
+            count[i] = 0u;
+            run += v[k];
+            if (i == n - 1u) start[n] = run;
+        }
+    }
+}
+
+__global__ __launch_bounds__(kTile) void k_scatter(const Tile* __restrict__ tiles, const uint32_t* __restrict__ key,
+                                                   const uint32_t* __restrict__ slot, const uint32_t* __restrict__ start,
+                                                   const uint32_t* __restrict__ perm, uint32_t* __restrict__ torig,
+                                                   uint32_t* __restrict__ tsrc, uint32_t* __restrict__ tkey)
+{
+    const Tile t = tiles[blockIdx.x];
+    const uint32_t s = t.begin + threadIdx.x;
+    if (s >= t.end) return;
+    const uint32_t k = key[s];
+    const uint32_t q = start[k] + slot[s];
+    torig[q] = perm[s];
+    tsrc[q] = s;
+    tkey[q] = k;
+}
+
+// Final slot = cell start + number of cell members with a smaller AoS index,
+// i.e. the order of a stable counting sort over AoS order, independent of the
+// atomic arrival order.  Gathers pos/vel from B into A.
+__global__ __launch_bounds__(kTile) void k_rank_reorder(const Tile* __restrict__ tiles, const uint32_t* __restrict__ start,
+                                                        const uint32_t* __restrict__ torig, const uint32_t* __restrict__ tsrc,
+                                                        const uint32_t* __restrict__ tkey, const float4* __restrict__ PB,
+                                                        const float4* __restrict__ VB, uint32_t* __restrict__ perm,
+                                                        float4* __restrict__ PA, float4* __restrict__ VA)
+{
+    const Tile t = tiles[blockIdx.x];
+    const uint32_t q = t.begin + threadIdx.x;
+    if (q >= t.end) return;
+    const uint32_t k = tkey[q];
+    const uint32_t b = start[k], e = start[k + 1];
+    const uint32_t o = torig[q];
+    uint32_t rank = 0;
+    for (uint32_t j = b; j < e; ++j) rank += (torig[j] < o) ? 1u : 0u;
+    const uint32_t dst = b + rank;
+    const uint32_t src = tsrc[q];
+    perm[dst] = o;
+    PA[dst] = PB[src];
+    VA[dst] = VB[src];
+}
+
+// ---------------------------------------------------------------- gathers ---
+
+// rho_pres_comp.glsl:35-58.  rho_i = m*315/(64 pi h^9) * sum_{r<h} (h^2-r^2)^3
+// over the 27-cell stencil (self included, :46); p_i = max(k(rho_i-rho0), 0).
+// The three x-adjacent cells of a stencil row are one contiguous slot range.
+__global__ __launch_bounds__(kTile) void k_density(const Tile* __restrict__ tiles, const SceneDev* __restrict__ scenes,
+                                                   const uint32_t* __restrict__ start, const float4* __restrict__ P,
+                                                   float* __restrict__ rho_out, float2* __restrict__ D2,
+                                                   const uint32_t* __restrict__ perm, sph_particle_t* __restrict__ aos)
+{
+    const Tile t = tiles[blockIdx.x];
+    const SceneDev& sc = scenes[t.scene];
+    const uint32_t s = t.begin + threadIdx.x;
+    if (s >= t.end) return;
+    const float4 xi = P[s];
+    const float h2 = sc.h2;
+    float acc = 0.f;
+    int cx, cy, cz;
+    if (cell_of(sc, xi.x, xi.y, xi.z, cx, cy, cz)) {
+        const int dx = sc.dims[0], dy = sc.dims[1], dz = sc.dims[2];
+        const int x0 = cx > 0 ? cx - 1 : 0;
+        const int x1 = cx < dx - 1 ? cx + 1 : dx - 1;
+        const int z0 = cz > 0 ? cz - 1 : 0, z1 = cz < dz - 1 ? cz + 1 : dz - 1;
+        const int y0 = cy > 0 ? cy - 1 : 0, y1 = cy < dy - 1 ? cy + 1 : dy - 1;
+        for (int z = z0; z <= z1; ++z) {
+            for (int y = y0; y <= y1; ++y) {
+                const uint32_t row = sc.cell_base + ((uint32_t)z * (uint32_t)dy + (uint32_t)y) * (uint32_t)dx;
+                const uint32_t b = start[row + (uint32_t)x0];
+                const uint32_t e = start[row + (uint32_t)x1 + 1u];
+                for (uint32_t j = b; j < e; ++j) {
+                    const float4 xj = P[j];
+                    const float ddx = xi.x - xj.x, ddy = xi.y - xj.y, ddz = xi.z - xj.z;
+                    const float r2 = fmaf(ddx, ddx, fmaf(ddy, ddy, ddz * ddz));
+                    const float w = h2 - r2;
+                    if (w > 0.f) acc = fmaf(w * w, w, acc);
+                }
+            }
+        }
+    }
+    const float rho = acc * sc.poly6_m;
+    const float p = fmaxf(sc.gas_const * (rho - sc.rho0), 0.f);
+    rho_out[s] = rho;
+    D2[s] = make_float2(p, 1.0f / rho);
+    if (aos) {
+        float* e = aos[perm[s]].extras;
+        e[0] = rho;
+        e[1] = p;
+    }
+}
+
+// force_comp.glsl:35-66 over the 27-cell stencil, j != i (:50-53), r < h (:57):
+//   pressure  += m(p_i+p_j)/(2 rho_j) * 45/(pi h^6) * (h-r)^2 * delta/r
+//   viscosity += m(v_j-v_i)/rho_j * 45/(pi h^6) * (h-r), then * visc
+//   F = pressure + viscosity + rho_i * G
+// FUSE additionally runs integrate_comp.glsl:35-81 on the new force, writes the
+// AoS record (pos, vel, force, extras.xy) and the next step's B state.
+template <bool FUSE>
+__global__ __launch_bounds__(kTile) void k_force(const Tile* __restrict__ tiles, const SceneDev* __restrict__ scenes,
+                                                 const uint32_t* __restrict__ start, const float4* __restrict__ P,
+                                                 const float4* __restrict__ V, const float* __restrict__ rho,
+                                                 const float2* __restrict__ D2, const uint32_t* __restrict__ perm,
+                                                 sph_particle_t* __restrict__ aos, float4* __restrict__ PN,
+                                                 float4* __restrict__ VN)
+{
+    const Tile t = tiles[blockIdx.x];
+    const SceneDev& sc = scenes[t.scene];
+    const uint32_t s = t.begin + threadIdx.x;
+    if (s >= t.end) return;
+    const float4 xi = P[s];
+    const float4 vi = V[s];
+    const float pi = D2[s].x;
+    const float rhoi = rho[s];
+    const float h = sc.h, h2 = sc.h2;
+    float ax = 0.f, ay = 0.f, az = 0.f;  // pressure sum
+    float bx = 0.f, by = 0.f, bz = 0.f;  // viscosity sum
+    int cx, cy, cz;
+    if (cell_of(sc, xi.x, xi.y, xi.z, cx, cy, cz)) {
+        const int dx = sc.dims[0], dy = sc.dims[1], dz = sc.dims[2];
+        const int x0 = cx > 0 ? cx - 1 : 0;
+        const int x1 = cx < dx - 1 ? cx + 1 : dx - 1;
+        const int z0 = cz > 0 ? cz - 1 : 0, z1 = cz < dz - 1 ? cz + 1 : dz - 1;
+        const int y0 = cy > 0 ? cy - 1 : 0, y1 = cy < dy - 1 ? cy + 1 : dy - 1;
+        for (int z = z0; z <= z1; ++z) {
+            for (int y = y0; y <= y1; ++y) {
+                const uint32_t row = sc.cell_base + ((uint32_t)z * (uint32_t)dy + (uint32_t)y) * (uint32_t)dx;
+                const uint32_t b = start[row + (uint32_t)x0];
+                const uint32_t e = start[row + (uint32_t)x1 + 1u];
+                for (uint32_t j = b; j < e; ++j) {
+                    if (j == s) continue;
+                    const float4 xj = P[j];
+                    const float ddx = xi.x - xj.x, ddy = xi.y - xj.y, ddz = xi.z - xj.z;
+                    const float r2 = fmaf(ddx, ddx, fmaf(ddy, ddy, ddz * ddz));
+                    if (r2 < h2) {
+                        const float inv_r = rsqrtf(r2);
+                        const float r = r2 * inv_r;
+                        const float q = h - r;
+                        const float2 dj = D2[j];
+                        const float4 vj = V[j];
+                        const float cp = (pi + dj.x) * dj.y * (q * q) * inv_r;
+                        ax = fmaf(cp, ddx, ax);
+                        ay = fmaf(cp, ddy, ay);
+                        az = fmaf(cp, ddz, az);
+                        const float cv = dj.y * q;
+                        bx = fmaf(cv, vj.x - vi.x, bx);
+                        by = fmaf(cv, vj.y - vi.y, by);
+                        bz = fmaf(cv, vj.z - vi.z, bz);
+                    }
+                }
+            }
+        }
+    }
+    float f[3];
+    f[0] = fmaf(sc.pres_m, ax, fmaf(sc.visc_m, bx, rhoi * sc.grav[0]));
+    f[1] = fmaf(sc.pres_m, ay, fmaf(sc.visc_m, by, rhoi * sc.grav[1]));
+    f[2] = fmaf(sc.pres_m, az, fmaf(sc.visc_m, bz, rhoi * sc.grav[2]));
+    float* r = reinterpret_cast<float*>(aos + perm[s]);
+    if constexpr (FUSE) {
+        const float x[3] = {xi.x, xi.y, xi.z}, v[3] = {vi.x, vi.y, vi.z};
+        float nx[3], nv[3];
+        integrate_one(sc, x, v, f, rhoi, nx, nv);
+        PN[s] = make_float4(nx[0], nx[1], nx[2], 0.f);
+        VN[s] = make_float4(nv[0], nv[1], nv[2], 0.f);
+        r[0] = nx[0];
+        r[1] = nx[1];
+        r[2] = nx[2];
+        r[4] = nv[0];
+        r[5] = nv[1];
+        r[6] = nv[2];
+        r[8] = f[0];
+        r[9] = f[1];
+        r[10] = f[2];
+        r[12] = rhoi;
+        r[13] = pi;
+    } else {
+        r[8] = f[0];
+        r[9] = f[1];
+        r[10] = f[2];
+    }
+}
+
+// integrate_comp.glsl:35-81 as a streaming pass over the AoS buffer.
+__global__ __launch_bounds__(kTile) void k_integrate_aos(const Tile* __restrict__ tiles, const SceneDev* __restrict__ scenes,
+                                                         sph_particle_t* __restrict__ aos)
+{
+    const Tile t = tiles[blockIdx.x];
+    const SceneDev& sc = scenes[t.scene];
+    const uint32_t i = t.begin + threadIdx.x;
+    if (i >= t.end) return;
+    float4* r = reinterpret_cast<float4*>(aos + i);
+    const float4 p = r[0], v = r[1], f4 = r[2], e = r[3];
+    const float x[3] = {p.x, p.y, p.z}, vel[3] = {v.x, v.y, v.z}, f[3] = {f4.x, f4.y, f4.z};
+    float nx[3], nv[3];
+    integrate_one(sc, x, vel, f, e.x, nx, nv);
+    r[0] = make_float4(nx[0], nx[1], nx[2], p.w);
+    r[1] = make_float4(nv[0], nv[1], nv[2], v.w);
+}
+
+// ------------------------------------------------------------- reductions ---
+
+// Per-tile partial sums over the AoS: rho, kinetic energy, position, max
+// speed, count of non-finite pos/vel.  Eight doubles per tile.
+__global__ __launch_bounds__(kTile) void k_metrics(const Tile* __restrict__ tiles, const SceneDev* __restrict__ scenes,
+                                                   const sph_particle_t* __restrict__ aos, double* __restrict__ partials,
+                                                   int first_tile)
+{
+    const int ti = first_tile + (int)blockIdx.x;
+    const Tile t = tiles[ti];
+    const SceneDev& sc = scenes[t.scene];
+    const uint32_t i = t.begin + threadIdx.x;
+    double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (i < t.end) {
+        const float4* r = reinterpret_cast<const float4*>(aos + i);
+        const float4 p = r[0], vel = r[1], e = r[3];
+        const double s2 = (double)vel.x * vel.x + (double)vel.y * vel.y + (double)vel.z * vel.z;
+        const bool finite = isfinite(p.x) && isfinite(p.y) && isfinite(p.z) && isfinite(vel.x) && isfinite(vel.y) &&
+                            isfinite(vel.z);
+        v[0] = e.x;
+        v[1] = 0.5 * (double)sc.mass * s2;
+        v[2] = p.x;
+        v[3] = p.y;
+        v[4] = p.z;
+        v[5] = finite ? sqrt(s2) : 0.0;
+        v[6] = finite ? 0.0 : 1.0;
+    }
+    __shared__ double red[8][kTile / 64];
+    const int lane = lane_id(), w = threadIdx.x / 64;
+#pragma unroll
+    for (int k = 0; k < 7; ++k) {
+        double x = v[k];
+        for (int d = 32; d > 0; d >>= 1) {
+            const double o = __shfl_xor(x, d);
+            x = (k == 5) ? fmax(x, o) : x + o;
+        }
+        if (lane == 0) red[k][w] = x;
+    }
+    __syncthreads();
+    if (threadIdx.x < 7) {
+        const int k = threadIdx.x;
+        double x = red[k][0];
+        for (int q = 1; q < kTile / 64; ++q) x = (k == 5) ? fmax(x, red[k][q]) : x + red[k][q];
+        partials[(size_t)ti * 8 + k] = x;
+    }
+}
+
+inline unsigned blocks_for(uint32_t n) { return (n + 255u) / 256u; }
+
+}  // namespace
+
+// --------------------------------------------------------------- launchers ---
+
+hipError_t launch_set_scene(hipStream_t s, SceneDev* dst, const SceneDev& value)
+{
+    hipLaunchKernelGGL(k_set_scene, dim3(1), dim3(1), 0, s, dst, value);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_aos_defaults(hipStream_t s, sph_particle_t* aos, uint32_t n)
+{
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fill_aos_defaults, dim3(blocks_for(n)), dim3(256), 0, s, aos, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_u32(hipStream_t s, uint32_t* p, uint32_t n, uint32_t value)
+{
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(n)), dim3(256), 0, s, p, n, value);
+    return hipGetLastError();
+}
+
+hipError_t launch_identity(hipStream_t s, uint32_t* p, uint32_t n)
+{
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_identity, dim3(blocks_for(n)), dim3(256), 0, s, p, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_lattice(hipStream_t s, sph_particle_t* aos, uint32_t first, const sph_lattice_t& spec)
+{
+    if (spec.count == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_lattice, dim3(blocks_for(spec.count)), dim3(256), 0, s, aos, first, spec);
+    return hipGetLastError();
+}
+
+hipError_t launch_refresh(hipStream_t s, const DeviceState& d, bool pos_vel, bool extras)
+{
+    hipLaunchKernelGGL(k_refresh, dim3(d.ntiles), dim3(kTile), 0, s, d.tiles, d.aos, d.perm, d.PB, d.VB, d.rho, d.D2,
+                       pos_vel ? 1 : 0, extras ? 1 : 0);
+    return hipGetLastError();
+}
+
+hipError_t launch_hash_count(hipStream_t s, const DeviceState& d)
+{
+    hipLaunchKernelGGL(k_hash_count, dim3(d.ntiles), dim3(kTile), 0, s, d.tiles, d.scenes, d.PB, d.key, d.slot, d.count);
+    return hipGetLastError();
+}
+
+hipError_t launch_scan_reduce(hipStream_t s, const DeviceState& d)
+{
+    hipLaunchKernelGGL(k_scan_reduce, dim3(scan_blocks(d.total_cells)), dim3(kScanThreads), 0, s, d.count,
+                       d.total_cells, d.bsum);
+    return hipGetLastError();
+}
+
+hipError_t launch_scan_top(hipStream_t s, const DeviceState& d)
+{
+    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, s, d.bsum, (uint32_t)scan_blocks(d.total_cells));
+    return hipGetLastError();
+}
+
+hipError_t launch_scan_down(hipStream_t s, const DeviceState& d)
+{
+    hipLaunchKernelGGL(k_scan_down, dim3(scan_blocks(d.total_cells)), dim3(kScanThreads), 0, s, d.count, d.total_cells,
+                       d.bsum, d.start);
+    return hipGetLastError();
+}
+
+hipError_t launch_scatter(hipStream_t s, const DeviceState& d)
+{
+    hipLaunchKernelGGL(k_scatter, dim3(d.ntiles), dim3(kTile), 0, s, d.tiles, d.key, d.slot, d.start, d.perm, d.torig,
+                       d.tsrc, d.tkey);
+    return hipGetLastError();
+}
+
+hipError_t launch_rank_reorder(hipStream_t s, const DeviceState& d)
+{
+    hipLaunchKernelGGL(k_rank_reorder, dim3(d.ntiles), dim3(kTile), 0, s, d.tiles, d.start, d.torig, d.tsrc, d.tkey,
+                       d.PB, d.VB, d.perm, d.PA, d.VA);
+    return hipGetLastError();
+}
+
+hipError_t launch_density(hipStream_t s, const DeviceState& d, bool write_aos)
+{
+    hipLaunchKernelGGL(k_density, dim3(d.ntiles), dim3(kTile), 0, s, d.tiles, d.scenes, d.start, d.PA, d.rho, d.D2,
+                       d.perm, write_aos ? d.aos : nullptr);
+    return hipGetLastError();
+}
+
+hipError_t launch_force(hipStream_t s, const DeviceState& d, bool fuse_integrate)
+{
+    if (fuse_integrate)
+        hipLaunchKernelGGL(k_force<true>, dim3(d.ntiles), dim3(kTile), 0, s, d.tiles, d.scenes, d.start, d.PA, d.VA,
+                           d.rho, d.D2, d.perm, d.aos, d.PB, d.VB);
+    else
+        hipLaunchKernelGGL(k_force<false>, dim3(d.ntiles), dim3(kTile), 0, s, d.tiles, d.scenes, d.start, d.PA, d.VA,
+                           d.rho, d.D2, d.perm, d.aos, d.PB, d.VB);
+    return hipGetLastError();
+}
+
+hipError_t launch_integrate_aos(hipStream_t s, const DeviceState& d)
+{
+    hipLaunchKernelGGL(k_integrate_aos, dim3(d.ntiles), dim3(kTile), 0, s, d.tiles, d.scenes, d.aos);
+    return hipGetLastError();
+}
+
+hipError_t launch_metrics(hipStream_t s, const DeviceState& d, int first_tile, int ntiles)
+{
+    if (ntiles <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_metrics, dim3(ntiles), dim3(kTile), 0, s, d.tiles, d.scenes, d.aos, d.partials, first_tile);
+    return hipGetLastError();
+}
+
+}  // namespace nprsph

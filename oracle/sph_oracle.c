@@ -1,0 +1,448 @@
+/*
+ * sph_oracle.c - CPU restatement of NPR-SPH's three compute passes.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the checker for the HIP solver in
+ * npr-sph_amd/: only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg load it.  The solver never calls it.
+ *
+ * What it restates (citations relative to the reference repository root):
+ *   orc_density_*   rho_pres_comp.glsl:35-58   (poly6 density, EOS pressure)
+ *   orc_forces_*    force_comp.glsl:35-66      (spiky pressure, viscosity, gravity)
+ *   orc_integrate   integrate_comp.glsl:35-81  (semi-implicit Euler, wall clamp)
+ *   orc_lattice     Main.cpp:488-539           (make_grid / init_particles, plus
+ *                                               seeded jitter, no reference counterpart)
+ *   orc_grid_*      no reference counterpart: the float32 cell hash and stable
+ *                   counting sort that the GPU neighbour build must equal bit
+ *                   for bit.
+ *
+ * Arithmetic: every pass runs in FP64 from the float32 inputs, keeping the
+ * shaders' literal operation order (length then `r < h`, pow(h^2-r^2,3) /
+ * pow(h,9), normalize = delta/r, lower wall tested before upper wall).
+ * PI is (double)3.141592741f (rho_pres_comp.glsl:8) and the smoothing length
+ * is the float32 product smoothing_coeff*PARTICLE_RADIUS (rho_pres_comp.glsl:40).
+ * The grid hash is float32 arithmetic with no contraction (-ffp-contract=off).
+ *
+ * Parity status: every function is pinned by tests/test_oracle.py (closed
+ * forms, invariants, brute force vs grid, an independent numpy lattice sum and
+ * the committed fixtures under tests/golden/).  The FP32 behaviour of the GL
+ * driver's built-ins is not reproducible here and is covered by tolerance.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+typedef struct { float pos[4], vel[4], force[4], extras[4]; } orc_particle;
+typedef struct { float mass, smoothing_coeff, visc, resting_rho; } orc_constants;
+typedef struct { float upper[4], lower[4]; } orc_boundary;
+typedef struct { float particle_radius, gas_const, gravity[3], damping, dt; } orc_fixed;
+
+typedef struct {
+    float lo[3];
+    float inv_cs;
+    int32_t dims[3];
+    uint32_t ncells; /* dims product; key `ncells` is the NaN cell */
+    double cs;
+} orc_grid;
+
+static const double ORC_PI = (double)3.141592741f; /* rho_pres_comp.glsl:8 */
+
+/* Smoothing length as the shader forms it, in float32 (rho_pres_comp.glsl:40). */
+float orc_smoothing_length(const orc_constants* c, const orc_fixed* f)
+{
+    float h = c->smoothing_coeff * f->particle_radius;
+    return h;
+}
+
+/* ---------------------------------------------------------------- grid --- */
+
+/* Cell edge = h*(1+2^-10) so that |xi-xj| < h always lands in adjacent cells
+ * despite float32 rounding of the hash; grown by 2^(1/4) steps until every
+ * axis has <= 2048 cells and the grid has <= max(65536, 8n) cells. */
+int orc_grid_geometry(float h, const orc_boundary* b, uint64_t n, orc_grid* g)
+{
+    if (!(h > 0.0f) || !isfinite(h)) return -1;
+    for (int a = 0; a < 3; ++a)
+        if (!isfinite(b->lower[a]) || !isfinite(b->upper[a])) return -1;
+    double cs = (double)h * (1.0 + 1.0 / 1024.0);
+    uint64_t cap = 8u * n;
+    if (cap < 65536u) cap = 65536u;
+    for (;;) {
+        uint64_t total = 1;
+        int ok = 1;
+        for (int a = 0; a < 3; ++a) {
+            double ext = (double)b->upper[a] - (double)b->lower[a];
+            double d = ceil(ext / cs);
+            if (!(d >= 1.0)) d = 1.0;
+            if (d > 2048.0) { ok = 0; d = 2048.0; }
+            g->dims[a] = (int32_t)d;
+            total *= (uint64_t)g->dims[a];
+        }
+        if (ok && total <= cap) { g->ncells = (uint32_t)total; break; }
+        cs *= 1.189207115002721; /* 2^(1/4) */
+    }
+    g->cs = cs;
+    g->inv_cs = (float)(1.0 / cs);
+    for (int a = 0; a < 3; ++a) g->lo[a] = b->lower[a];
+    return 0;
+}
+
+/* float32 cell hash: coordinate c = clamp(floor((x-lo)*inv_cs), 0, dim-1),
+ * key = (cz*dy + cy)*dx + cx; any NaN component gives key = ncells. */
+static inline uint32_t orc_key(const orc_grid* g, float x, float y, float z)
+{
+    float fx = (x - g->lo[0]) * g->inv_cs;
+    float fy = (y - g->lo[1]) * g->inv_cs;
+    float fz = (z - g->lo[2]) * g->inv_cs;
+    if (fx != fx || fy != fy || fz != fz) return g->ncells;
+    int32_t cx = fx < 0.0f ? 0 : (fx >= (float)g->dims[0] ? g->dims[0] - 1 : (int32_t)fx);
+    int32_t cy = fy < 0.0f ? 0 : (fy >= (float)g->dims[1] ? g->dims[1] - 1 : (int32_t)fy);
+    int32_t cz = fz < 0.0f ? 0 : (fz >= (float)g->dims[2] ? g->dims[2] - 1 : (int32_t)fz);
+    return ((uint32_t)cz * (uint32_t)g->dims[1] + (uint32_t)cy) * (uint32_t)g->dims[0] + (uint32_t)cx;
+}
+
+uint32_t orc_cell_key(const orc_grid* g, float x, float y, float z) { return orc_key(g, x, y, z); }
+
+/* Stable counting sort of n particles by cell key.  pos points at the x of
+ * particle 0, consecutive particles are `stride` floats apart.
+ * keys[n]; cell_start[ncells+2] (cells 0..ncells, the last being the NaN
+ * cell, then n); perm[slot] = particle index, ordered by (key, index). */
+void orc_grid_build(uint64_t n, const float* pos, uint64_t stride, const orc_grid* g,
+                    uint32_t* keys, uint32_t* cell_start, uint32_t* perm)
+{
+    const uint64_t nc = (uint64_t)g->ncells + 1;
+    uint32_t* cnt = (uint32_t*)calloc(nc + 1, sizeof(uint32_t));
+    for (uint64_t i = 0; i < n; ++i) {
+        const float* p = pos + i * stride;
+        uint32_t k = orc_key(g, p[0], p[1], p[2]);
+        keys[i] = k;
+        cnt[k]++;
+    }
+    uint32_t run = 0;
+    for (uint64_t c = 0; c < nc; ++c) {
+        cell_start[c] = run;
+        run += cnt[c];
+        cnt[c] = cell_start[c];
+    }
+    cell_start[nc] = run;
+    for (uint64_t i = 0; i < n; ++i) perm[cnt[keys[i]]++] = (uint32_t)i;
+    free(cnt);
+}
+
+/* ------------------------------------------------------- neighbour walk --- */
+
+typedef struct {
+    orc_grid g;
+    uint32_t* keys;
+    uint32_t* start;
+    uint32_t* perm;
+} orc_cells;
+
+static int orc_cells_build(orc_cells* cl, uint64_t n, const double* x, float h, const orc_boundary* b)
+{
+    if (orc_grid_geometry(h, b, n, &cl->g) != 0) return -1;
+    float* xf = (float*)malloc(sizeof(float) * 3 * (n ? n : 1));
+    for (uint64_t i = 0; i < 3 * n; ++i) xf[i] = (float)x[i];
+    cl->keys = (uint32_t*)malloc(sizeof(uint32_t) * (n ? n : 1));
+    cl->start = (uint32_t*)malloc(sizeof(uint32_t) * ((uint64_t)cl->g.ncells + 2));
+    cl->perm = (uint32_t*)malloc(sizeof(uint32_t) * (n ? n : 1));
+    orc_grid_build(n, xf, 3, &cl->g, cl->keys, cl->start, cl->perm);
+    free(xf);
+    return 0;
+}
+
+static void orc_cells_free(orc_cells* cl)
+{
+    free(cl->keys);
+    free(cl->start);
+    free(cl->perm);
+}
+
+/* Visit the 3x3x3 cell stencil around particle i's cell.  The NaN cell has no
+ * neighbours, as in the reference, where every comparison with NaN is false. */
+#define ORC_FOR_NEIGHBOURS(cl, i, J, ...)                                                    \
+    do {                                                                                     \
+        uint32_t key_ = (cl)->keys[i];                                                       \
+        if (key_ < (cl)->g.ncells) {                                                         \
+            int32_t dx_ = (cl)->g.dims[0], dy_ = (cl)->g.dims[1], dz_ = (cl)->g.dims[2];     \
+            int32_t cx_ = (int32_t)(key_ % (uint32_t)dx_);                                   \
+            int32_t cy_ = (int32_t)((key_ / (uint32_t)dx_) % (uint32_t)dy_);                 \
+            int32_t cz_ = (int32_t)(key_ / ((uint32_t)dx_ * (uint32_t)dy_));                 \
+            for (int32_t z_ = cz_ - 1; z_ <= cz_ + 1; ++z_) {                                \
+                if (z_ < 0 || z_ >= dz_) continue;                                           \
+                for (int32_t y_ = cy_ - 1; y_ <= cy_ + 1; ++y_) {                            \
+                    if (y_ < 0 || y_ >= dy_) continue;                                       \
+                    int32_t x0_ = cx_ > 0 ? cx_ - 1 : 0;                                     \
+                    int32_t x1_ = cx_ < dx_ - 1 ? cx_ + 1 : dx_ - 1;                         \
+                    uint32_t row_ = ((uint32_t)z_ * (uint32_t)dy_ + (uint32_t)y_) * (uint32_t)dx_; \
+                    for (uint32_t s_ = (cl)->start[row_ + x0_]; s_ < (cl)->start[row_ + x1_ + 1]; ++s_) { \
+                        uint64_t J = (cl)->perm[s_];                                         \
+                        __VA_ARGS__                                                          \
+                    }                                                                        \
+                }                                                                            \
+            }                                                                                \
+        }                                                                                    \
+    } while (0)
+
+/* --------------------------------------------------------------- passes --- */
+
+/* One density/pressure term pair, rho_pres_comp.glsl:48-52. */
+static inline double orc_poly6_term(double m, double h, double r)
+{
+    return m * 315.0 * pow(h * h - r * r, 3) / (64.0 * ORC_PI * pow(h, 9));
+}
+
+/* rho_pres_comp.glsl:35-58, O(N^2) over all j (self included, :46).
+ * x: 3n doubles.  Out: rho[n], p[n]. */
+void orc_density_bf(uint64_t n, const double* x, const orc_constants* c, const orc_fixed* f,
+                    double* rho, double* p)
+{
+    const double h = (double)orc_smoothing_length(c, f);
+    const double m = (double)c->mass;
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < (int64_t)n; ++i) {
+        double acc = 0.0;
+        for (uint64_t j = 0; j < n; ++j) {
+            double d0 = x[3 * i] - x[3 * j], d1 = x[3 * i + 1] - x[3 * j + 1], d2 = x[3 * i + 2] - x[3 * j + 2];
+            double r = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+            if (r < h) acc += orc_poly6_term(m, h, r);
+        }
+        rho[i] = acc;
+        double pr = (double)f->gas_const * (acc - (double)c->resting_rho);
+        p[i] = pr > 0.0 ? pr : 0.0; /* max(., 0), :58 */
+    }
+}
+
+/* Same sums restricted to the 27-cell stencil of a cell list built from the
+ * float32 image of x; the cell edge is >= h, so the neighbour set is exact. */
+int orc_density_grid(uint64_t n, const double* x, const orc_constants* c, const orc_fixed* f,
+                     const orc_boundary* b, double* rho, double* p)
+{
+    const float hf = orc_smoothing_length(c, f);
+    const double h = (double)hf;
+    const double m = (double)c->mass;
+    orc_cells cl;
+    if (orc_cells_build(&cl, n, x, hf, b) != 0) return -1;
+#pragma omp parallel for schedule(dynamic, 256)
+    for (int64_t i = 0; i < (int64_t)n; ++i) {
+        double acc = 0.0;
+        ORC_FOR_NEIGHBOURS(&cl, i, j, {
+            double d0 = x[3 * i] - x[3 * j], d1 = x[3 * i + 1] - x[3 * j + 1], d2 = x[3 * i + 2] - x[3 * j + 2];
+            double r = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+            if (r < h) acc += orc_poly6_term(m, h, r);
+        });
+        rho[i] = acc;
+        double pr = (double)f->gas_const * (acc - (double)c->resting_rho);
+        p[i] = pr > 0.0 ? pr : 0.0;
+    }
+    orc_cells_free(&cl);
+    return 0;
+}
+
+/* force_comp.glsl:48-61 body for one pair.  Accumulates the pressure and
+ * viscosity terms and the sum of their magnitudes (for error bounds). */
+static inline void orc_pair_force(uint64_t i, uint64_t j, const double* x, const double* v,
+                                  const double* rho, const double* p, double m, double h,
+                                  double spiky, double lap, double visc,
+                                  double* fp, double* fv, double* mag)
+{
+    double d0 = x[3 * i] - x[3 * j], d1 = x[3 * i + 1] - x[3 * j + 1], d2 = x[3 * i + 2] - x[3 * j + 2];
+    double r = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+    if (!(r < h)) return; /* :57 */
+    /* normalize(delta) = delta / length(delta); NaN at r == 0, as GLSL. */
+    double n0 = d0 / r, n1 = d1 / r, n2 = d2 / r;
+    double cp = m * (p[i] + p[j]) / (2.0 * rho[j]) * spiky * pow(h - r, 2);
+    fp[0] -= cp * n0; /* :59 */
+    fp[1] -= cp * n1;
+    fp[2] -= cp * n2;
+    double cv = m / rho[j] * lap * (h - r);
+    double w0 = (v[3 * j] - v[3 * i]) * cv, w1 = (v[3 * j + 1] - v[3 * i + 1]) * cv, w2 = (v[3 * j + 2] - v[3 * i + 2]) * cv;
+    fv[0] += w0; /* :60 */
+    fv[1] += w1;
+    fv[2] += w2;
+    *mag += fabs(cp) + visc * sqrt(w0 * w0 + w1 * w1 + w2 * w2);
+}
+
+static inline void orc_finish_force(uint64_t i, const double* rho, const orc_fixed* f, double visc,
+                                     const double* fp, const double* fv, double mag,
+                                     double* force, double* scale)
+{
+    double g0 = (double)f->gravity[0], g1 = (double)f->gravity[1], g2 = (double)f->gravity[2];
+    /* visc_force *= visc (:63); F = pres + visc + rho_i*G (:65-66) */
+    force[3 * i] = fp[0] + fv[0] * visc + rho[i] * g0;
+    force[3 * i + 1] = fp[1] + fv[1] * visc + rho[i] * g1;
+    force[3 * i + 2] = fp[2] + fv[2] * visc + rho[i] * g2;
+    if (scale) scale[i] = mag + fabs(rho[i]) * sqrt(g0 * g0 + g1 * g1 + g2 * g2);
+}
+
+/* force_comp.glsl:35-66, O(N^2) over j != i (:50-53).
+ * Out: force[3n]; scale[n] (may be NULL) = sum over terms of |term|, the
+ * natural bound for floating-point error of the sum. */
+void orc_forces_bf(uint64_t n, const double* x, const double* v, const double* rho, const double* p,
+                   const orc_constants* c, const orc_fixed* f, double* force, double* scale)
+{
+    const double h = (double)orc_smoothing_length(c, f);
+    const double m = (double)c->mass, visc = (double)c->visc;
+    const double spiky = -45.0 / (ORC_PI * pow(h, 6)); /* :41 */
+    const double lap = 45.0 / (ORC_PI * pow(h, 6));    /* :42 */
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < (int64_t)n; ++i) {
+        double fp[3] = {0, 0, 0}, fv[3] = {0, 0, 0}, mag = 0.0;
+        for (uint64_t j = 0; j < n; ++j) {
+            if ((uint64_t)i == j) continue;
+            orc_pair_force((uint64_t)i, j, x, v, rho, p, m, h, spiky, lap, visc, fp, fv, &mag);
+        }
+        orc_finish_force((uint64_t)i, rho, f, visc, fp, fv, mag, force, scale);
+    }
+}
+
+int orc_forces_grid(uint64_t n, const double* x, const double* v, const double* rho, const double* p,
+                    const orc_constants* c, const orc_fixed* f, const orc_boundary* b,
+                    double* force, double* scale)
+{
+    const float hf = orc_smoothing_length(c, f);
+    const double h = (double)hf;
+    const double m = (double)c->mass, visc = (double)c->visc;
+    const double spiky = -45.0 / (ORC_PI * pow(h, 6));
+    const double lap = 45.0 / (ORC_PI * pow(h, 6));
+    orc_cells cl;
+    if (orc_cells_build(&cl, n, x, hf, b) != 0) return -1;
+#pragma omp parallel for schedule(dynamic, 256)
+    for (int64_t i = 0; i < (int64_t)n; ++i) {
+        double fp[3] = {0, 0, 0}, fv[3] = {0, 0, 0}, mag = 0.0;
+        ORC_FOR_NEIGHBOURS(&cl, i, j, {
+            if ((uint64_t)i != j)
+                orc_pair_force((uint64_t)i, j, x, v, rho, p, m, h, spiky, lap, visc, fp, fv, &mag);
+        });
+        orc_finish_force((uint64_t)i, rho, f, visc, fp, fv, mag, force, scale);
+    }
+    orc_cells_free(&cl);
+    return 0;
+}
+
+/* integrate_comp.glsl:35-81, in place on x and v.
+ * near_wall[i] (may be NULL) gets bit a set when the unclamped new position
+ * on axis a lies within `wall_eps` of a wall, where float32 and FP64 may take
+ * different branches of the clamp. */
+void orc_integrate(uint64_t n, double* x, double* v, const double* force, const double* rho,
+                   const orc_boundary* b, const orc_fixed* f, double wall_eps, uint8_t* near_wall)
+{
+    const double dt = (double)f->dt, damp = (double)f->damping;
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < (int64_t)n; ++i) {
+        uint8_t flags = 0;
+        for (int a = 0; a < 3; ++a) {
+            double acc = force[3 * i + a] / rho[i];    /* :41 */
+            double nv = v[3 * i + a] + dt * acc;       /* :42 */
+            double np = x[3 * i + a] + dt * nv;        /* :43 */
+            double lo = (double)b->lower[a], hi = (double)b->upper[a];
+            if (fabs(np - lo) <= wall_eps || fabs(np - hi) <= wall_eps) flags |= (uint8_t)(1u << a);
+            if (np < lo) {                             /* :46-55 per axis */
+                np = lo;
+                nv *= -damp;
+            } else if (np > hi) {
+                np = hi;
+                nv *= -damp;
+            }
+            x[3 * i + a] = np;
+            v[3 * i + a] = nv;
+        }
+        if (near_wall) near_wall[i] = flags;
+    }
+}
+
+/* One frame of Main.cpp:295-303 on FP64 state (grid or brute force). */
+int orc_step(uint64_t n, double* x, double* v, double* force, double* rho, double* p,
+             const orc_constants* c, const orc_boundary* b, const orc_fixed* f, int use_grid)
+{
+    if (use_grid) {
+        if (orc_density_grid(n, x, c, f, b, rho, p) != 0) return -1;
+        if (orc_forces_grid(n, x, v, rho, p, c, f, b, force, NULL) != 0) return -1;
+    } else {
+        orc_density_bf(n, x, c, f, rho, p);
+        orc_forces_bf(n, x, v, rho, p, c, f, force, NULL);
+    }
+    orc_integrate(n, x, v, force, rho, b, f, 0.0, NULL);
+    return 0;
+}
+
+/* ----------------------------------------------------------- generators --- */
+
+/* SplitMix64 output function (Steele, Lea and Flood, OOPSLA 2014), used as a
+ * counter-based generator: value(seed, k) = mix(seed + (k+1)*golden). */
+uint64_t orc_splitmix64(uint64_t seed, uint64_t counter)
+{
+    uint64_t z = seed + (counter + 1u) * 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+/* Jitter in [-amp, amp): top 24 bits, centred, scaled by amp*2^-23 (one
+ * float32 rounding). */
+static inline float orc_jitter(uint64_t seed, uint64_t counter, float amp)
+{
+    int32_t u = (int32_t)(orc_splitmix64(seed, counter) >> 40) - (1 << 23);
+    float scale = amp * (1.0f / 8388608.0f);
+    return (float)u * scale;
+}
+
+/* Seeded jittered lattice; see sph_lattice_t in include/sph.h.
+ * Writes `count` particles. */
+void orc_lattice(int32_t nx, int32_t ny, int32_t nz, uint32_t count, const float* origin, float spacing,
+                 float jitter, uint64_t seed, orc_particle* out)
+{
+    (void)nx;
+    for (uint32_t k = 0; k < count; ++k) {
+        uint32_t iz = k % (uint32_t)nz;
+        uint32_t iy = (k / (uint32_t)nz) % (uint32_t)ny;
+        uint32_t ix = k / ((uint32_t)nz * (uint32_t)ny);
+        uint32_t idx[3] = {ix, iy, iz};
+        orc_particle* p = &out[k];
+        memset(p, 0, sizeof(*p));
+        for (int a = 0; a < 3; ++a) {
+            float base = origin[a] + (float)idx[a] * spacing;
+            p->pos[a] = base + orc_jitter(seed, 3u * (uint64_t)k + (uint64_t)a, jitter);
+        }
+        p->pos[3] = 1.0f;
+    }
+}
+
+/* make_grid of Main.cpp:488-505: 10x100x10 at spacing PARTICLE_RADIUS from the
+ * origin, index = 1000*i + 10*j + k, no jitter. */
+void orc_reference_grid(orc_particle* out)
+{
+    const float pr = 0.005f; /* PARTICLE_RADIUS, Main.cpp:34 */
+    int idx = 0;
+    for (int i = 0; i < 10; i++)
+        for (int j = 0; j < 100; j++)
+            for (int k = 0; k < 10; k++) {
+                orc_particle* p = &out[idx++];
+                memset(p, 0, sizeof(*p));
+                p->pos[0] = (float)i * pr;
+                p->pos[1] = (float)j * pr;
+                p->pos[2] = (float)k * pr;
+                p->pos[3] = 1.0f;
+            }
+}
+
+int orc_num_threads(void)
+{
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
+void orc_set_threads(int t)
+{
+#ifdef _OPENMP
+    if (t > 0) omp_set_num_threads(t);
+#else
+    (void)t;
+#endif
+}

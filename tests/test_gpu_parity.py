@@ -1,0 +1,180 @@
+"""GPU parity of the three passes against the FP64 oracle, through the C ABI.
+
+Each pass is compared from the identical float32 input state:
+  rho_pres  vs rho_pres_comp.glsl:35-58   |drho| <= 2e-5 rho
+  force     vs force_comp.glsl:35-66      |dF|   <= 2e-5 * sum_j |term_ij| (oracle `scale`)
+  integrate vs integrate_comp.glsl:35-81  |dv|, |dx| <= 4 ulp-scale of their operands; axes
+            whose FP64 result lies within 1e-6 of a wall may take either clamp branch.
+The neighbour grid (cell starts and slot permutation) must equal the CPU
+build bit for bit, and a graph-captured step must equal the three passes bit
+for bit.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+RHO_RTOL = 2e-5
+FORCE_TOL = 2e-5
+F32_EPS = np.float64(np.finfo(np.float32).eps)
+
+
+def _lattice(orc, sc, count=None):
+    return orc.lattice(sc.nx, sc.ny, sc.nz, sc.spacing, sc.origin, sc.jitter, sc.seed,
+                       count=sc.count if count is None else count)
+
+
+def check_density(orc, before, after, ob, grid=False, c=None, f=None):
+    rho, pr = orc.density(orc.pos64(before), c=c, f=f, b=ob, grid=grid)
+    g_rho = after["extras"][:, 0].astype(np.float64)
+    g_p = after["extras"][:, 1].astype(np.float64)
+    err = np.abs(g_rho - rho)
+    assert np.all(err <= RHO_RTOL * rho + 1e-30), f"rho rel err {np.max(err / np.maximum(rho, 1e-30))}"
+    k = 2000.0 if f is None else float(f.gas_const)
+    perr = np.abs(g_p - pr)
+    assert np.all(perr <= k * RHO_RTOL * rho + 4 * F32_EPS * pr + 1e-30), "pressure mismatch"
+
+
+def check_force(orc, state, after, ob, grid=False, c=None, f=None):
+    frc, scale = orc.forces(orc.pos64(state), orc.vel64(state), state["extras"][:, 0].astype(np.float64),
+                            state["extras"][:, 1].astype(np.float64), c=c, f=f, b=ob, grid=grid)
+    g = after["force"][:, :3].astype(np.float64)
+    err = np.sqrt(((g - frc) ** 2).sum(axis=1))
+    assert np.all(err <= FORCE_TOL * scale + 1e-30), f"force rel err {np.max(err / scale)}"
+
+
+def check_integrate(orc, state, after, ob, f=None):
+    x, v, near = orc.integrate(orc.pos64(state), orc.vel64(state), state["force"][:, :3],
+                               state["extras"][:, 0], b=ob, f=f, wall_eps=1e-6)
+    gx = after["pos"][:, :3].astype(np.float64)
+    gv = after["vel"][:, :3].astype(np.float64)
+    dt = float(np.float32(1e-4)) if f is None else float(f.dt)
+    acc = state["force"][:, :3].astype(np.float64) / state["extras"][:, [0]].astype(np.float64)
+    vscale = np.abs(state["vel"][:, :3]) + dt * np.abs(acc) + np.abs(v)
+    xscale = np.abs(state["pos"][:, :3]) + dt * np.abs(v) + np.abs(x)
+    strict = np.ones_like(gx, dtype=bool)
+    for a in range(3):
+        strict[:, a] = (near & (1 << a)) == 0
+    assert np.all(np.abs(gv - v)[strict] <= 4 * F32_EPS * vscale[strict] + 1e-30), "velocity mismatch"
+    assert np.all(np.abs(gx - x)[strict] <= 4 * F32_EPS * xscale[strict] + 1e-30), "position mismatch"
+    # Near a wall either branch is acceptable, but the result must be one of them.
+    if (~strict).any():
+        lo = np.array(ob.lower[:3], dtype=np.float64)
+        hi = np.array(ob.upper[:3], dtype=np.float64)
+        on_wall = np.isclose(gx, lo[None, :], atol=0, rtol=0) | np.isclose(gx, hi[None, :], atol=0, rtol=0)
+        ok = np.abs(gx - x) <= 1e-5 + 4 * F32_EPS * xscale
+        assert np.all((ok | on_wall)[~strict])
+
+
+def test_passes_dam10k_vs_bruteforce(pkg, orc, gpu_lib):
+    """dam10k: each pass equals the O(N^2) FP64 transcription (the reference's own algorithm)."""
+    sc = pkg.scenes.dam10k()
+    p0 = _lattice(orc, sc)
+    b = pkg.sph.make_boundary(sc.lower, sc.upper)
+    ob = orc.make_boundary(sc.lower, sc.upper)
+    with pkg.Simulation(sc.count, boundaries=b) as sim:
+        sim.upload(p0)
+        sim.rho_pres()
+        s1 = sim.download()
+        check_density(orc, p0, s1, ob, grid=False)
+        sim.force()
+        s2 = sim.download()
+        check_force(orc, s1, s2, ob, grid=False)
+        sim.integrate()
+        s3 = sim.download()
+        check_integrate(orc, s2, s3, ob)
+        # Untouched lanes are preserved bit for bit.
+        for name, lane in (("pos", 3), ("vel", 3), ("force", 3), ("extras", 2), ("extras", 3)):
+            assert np.array_equal(s3[name][:, lane].view(np.uint32), p0[name][:, lane].view(np.uint32))
+
+
+def test_passes_after_steps_dam10k(pkg, orc, gpu_lib):
+    """Parity holds from an evolved (disordered, wall-touching) state too."""
+    sc = pkg.scenes.dam10k()
+    p0 = _lattice(orc, sc)
+    b = pkg.sph.make_boundary(sc.lower, sc.upper)
+    ob = orc.make_boundary(sc.lower, sc.upper)
+    with pkg.Simulation(sc.count, boundaries=b) as sim:
+        sim.upload(p0)
+        sim.step(200)
+        st = sim.download()
+        sim.rho_pres()
+        s1 = sim.download()
+        check_density(orc, st, s1, ob, grid=False)
+        sim.force()
+        s2 = sim.download()
+        check_force(orc, s1, s2, ob, grid=False)
+        sim.integrate()
+        s3 = sim.download()
+        check_integrate(orc, s2, s3, ob)
+
+
+def test_grid_bit_exact(pkg, orc, gpu_lib):
+    """Cell starts and the (cell, AoS index) slot order equal the CPU stable counting sort."""
+    sc = pkg.scenes.dam10k()
+    p0 = _lattice(orc, sc)
+    b = pkg.sph.make_boundary(sc.lower, sc.upper)
+    ob = orc.make_boundary(sc.lower, sc.upper)
+    with pkg.Simulation(sc.count, boundaries=b) as sim:
+        for nsteps in (0, 150):
+            sim.upload(p0)
+            if nsteps:
+                sim.step(nsteps)
+            st = sim.download()
+            sim.rho_pres()
+            start, perm = sim.download_grid()
+            gi = sim.grid_info(0)
+            g = orc.grid_geometry(orc.smoothing_length(), ob, sc.count)
+            assert tuple(gi.dims) == tuple(g.dims) and gi.ncells == g.ncells
+            assert np.float32(gi.inv_cell) == np.float32(g.inv_cs)
+            keys, cstart, cperm = orc.grid_build(st["pos"][:, :3], g)
+            assert np.array_equal(start, cstart)
+            assert np.array_equal(perm, cperm)
+
+
+def test_fused_step_equals_passes(pkg, orc, gpu_lib):
+    sc = pkg.scenes.dam10k()
+    p0 = _lattice(orc, sc)
+    b = pkg.sph.make_boundary(sc.lower, sc.upper)
+    with pkg.Simulation(sc.count, boundaries=b) as a, pkg.Simulation(sc.count, boundaries=b) as c:
+        a.upload(p0)
+        c.upload(p0)
+        a.step(37)  # 2 x 16-step graph + 5 single-step graphs
+        for _ in range(37):
+            c.rho_pres()
+            c.force()
+            c.integrate()
+        assert a.download().tobytes() == c.download().tobytes()
+
+
+def test_device_lattice_matches_cpu(pkg, orc, gpu_lib):
+    sc = pkg.scenes.dam10k()
+    lat = pkg.sph.make_lattice(**sc.lattice_args())
+    with pkg.Simulation(sc.count) as sim:
+        sim.init_lattice(lat)
+        got = sim.download()
+    assert got.tobytes() == _lattice(orc, sc).tobytes()
+
+
+def test_dam1m_one_step_vs_grid_oracle(pkg, orc, gpu_lib):
+    """1M particles: per-pass parity against the FP64 grid transcription."""
+    sc = pkg.scenes.dam1m()
+    b = pkg.sph.make_boundary(sc.lower, sc.upper)
+    ob = orc.make_boundary(sc.lower, sc.upper)
+    with pkg.Simulation(sc.count, boundaries=b) as sim:
+        sim.init_lattice(pkg.sph.make_lattice(**sc.lattice_args()))
+        sim.step(20)
+        st = sim.download()
+        sim.rho_pres()
+        s1 = sim.download()
+        check_density(orc, st, s1, ob, grid=True)
+        sim.force()
+        s2 = sim.download()
+        check_force(orc, s1, s2, ob, grid=True)
+        sim.integrate()
+        s3 = sim.download()
+        check_integrate(orc, s2, s3, ob)
+        start, perm = sim.download_grid()
+        g = orc.grid_geometry(orc.smoothing_length(), ob, sc.count)
+        _, cstart, cperm = orc.grid_build(s2["pos"][:, :3], g)
+        assert np.array_equal(start, cstart) and np.array_equal(perm, cperm)

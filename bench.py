@@ -1,0 +1,177 @@
+"""Throughput benchmark of the MI355X SPH solver (one GPU).
+
+A step is one frame of NPR-SPH's simulation (Main.cpp:291-304): neighbour
+grid build + rho_pres + force + integrate over every particle.  The default
+workload is dam1m: 1,000,000 particles, a jittered 100^3 dam-break lattice at
+spacing 0.5h with the reference's constants, initialised on the device.
+
+Printed JSON line (one):
+  value        particle-steps/s over the timed steps (inputs resident in HBM)
+  roofline     the dominant kernel: algorithmic bytes per launch / its mean
+               duration (HIP events on the solver's stream) vs 8 TB/s
+  cpu_baseline the FP64 oracle (grid form, OpenMP) on this host, on a bounded
+               sample of the same workload
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+import __graft_entry__ as entry  # noqa: E402
+
+METRIC = "particle-steps/s at 1M particles, plain vs learned coarse step; % HBM roofline"
+HBM_PEAK_GBS = 8000.0
+# Algorithmic bytes per particle: compulsory float32 reads + writes of each pass.
+#   density          pos.xyz 12 read; rho, p 8 written                      = 20
+#   force            pos, vel 24 + rho, p 8 read; force.xyz 12 written      = 44
+#   integrate        pos, vel, force 36 + rho 4 read; pos, vel 24 written   = 64
+#   force_integrate  the fused pair: pos, vel, rho, p 32 read;
+#                    force, pos, vel 36 written                             = 68
+ALGO_BYTES = {"density": 20, "force": 44, "integrate": 64, "force_integrate": 68}
+STEP_BYTES = 20 + 44 + 64  # the three reference passes, 128 B per particle-step
+
+
+def _scene(pkg, name):
+    sc = pkg.scenes
+    if name in sc.WORKLOADS:
+        return [sc.WORKLOADS[name]()]
+    if name == "batch256":
+        return sc.batch256()
+    raise SystemExit(f"unknown workload {name}")
+
+
+def make_sim(pkg, scenes):
+    sph = pkg.sph
+    sim = pkg.Simulation([s.count for s in scenes], boundaries=[sph.make_boundary(s.lower, s.upper) for s in scenes])
+    for i, s in enumerate(scenes):
+        sim.init_lattice(sph.make_lattice(**s.lattice_args()), scene=i)
+    return sim
+
+
+def cpu_baseline(pkg, scene, budget_s: float):
+    """FP64 grid transcription (oracle) on this host's cores, bounded sample."""
+    orc = entry.load_oracle()
+    threads = min(16, os.cpu_count() or 1)
+    orc.set_threads(threads)
+    p0 = orc.lattice(scene.nx, scene.ny, scene.nz, scene.spacing, scene.origin, scene.jitter, scene.seed,
+                     count=scene.count)
+    ob = orc.make_boundary(scene.lower, scene.upper)
+    st = orc.State64(p0)
+    t0 = time.perf_counter()
+    st.step(b=ob, grid=True)  # warm-up step
+    t1 = time.perf_counter()
+    per = max(t1 - t0, 1e-6)
+    nsteps = max(1, min(200, int(budget_s / per)))
+    t0 = time.perf_counter()
+    st.step(b=ob, grid=True, nsteps=nsteps)
+    el = time.perf_counter() - t0
+    return {"value": scene.count * nsteps / el, "unit": "particle-steps/s", "cores": threads, "kind": "port",
+            "sample": f"{scene.count} particles, {nsteps} steps of the FP64 grid transcription after 1 warm-up "
+                      f"step, {threads} OpenMP threads ({el:.1f} s)"}
+
+
+def copy_probe(mbytes: int, reps: int = 20):
+    """Plain device copy of known size (calibrates rocprof FETCH/WRITE counters)."""
+    import torch
+
+    n = mbytes * (1 << 20) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    print(json.dumps({"copy_probe_MiB": mbytes, "GBps_rd_wr": 2 * n * 4 * reps / el / 1e9}))
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--workload", default="dam1m")
+    ap.add_argument("--timing-steps", type=int, default=20, help="steps run with per-kernel HIP events")
+    ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU oracle work (0 = skip)")
+    ap.add_argument("--copy-probe", type=int, default=0, help="MiB: run only a plain copy kernel")
+    args = ap.parse_args(argv)
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (each GPU host here has one GPU, "
+              "and the solver shards by spatial slab only in design)", file=sys.stderr)
+        return 2
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return 0
+
+    pkg = entry.load_package()
+    scenes = _scene(pkg, args.workload)
+    n = sum(s.count for s in scenes)
+    sim = make_sim(pkg, scenes)
+    sim.step(args.warmup)
+    sim.synchronize()
+    t0 = time.perf_counter()
+    sim.step(args.steps)
+    sim.synchronize()
+    elapsed = time.perf_counter() - t0
+    ms_per_step = 1e3 * elapsed / args.steps
+    value = n * args.steps / elapsed
+
+    # Per-kernel durations, HIP events on the solver's stream (ungraphed launches).
+    sim.set_timing(True)
+    sim.step(args.timing_steps)
+    times = sim.kernel_times()
+    sim.set_timing(False)
+    kern = {k: {"ms_avg": v[0] / v[1], "launches": v[1]} for k, v in times.items()}
+    dom = max(times, key=lambda k: times[k][0])
+    dom_ms = times[dom][0] / times[dom][1]
+    algo = ALGO_BYTES.get(dom)
+    roofline = None
+    if algo is not None:
+        achieved = algo * n / (dom_ms * 1e-3) / 1e9
+        roofline = {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": achieved / HBM_PEAK_GBS, "traffic": None, "algo_bytes_per_particle": algo,
+                    "launch_ms": dom_ms}
+    m = sim.metrics(0)
+    sim.close()
+
+    cpu = None
+    if args.cpu_budget > 0:
+        cpu = cpu_baseline(pkg, scenes[0], args.cpu_budget)
+        if len(scenes) > 1:
+            cpu["sample"] += " (first scene of the batch)"
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": args.workload, "particles": n, "scenes": len(scenes), "form": "plain",
+                   "dt": 1e-4, "spacing": "0.5h jittered lattice", "steps_from_initial_state": args.warmup},
+        "step_hbm_frac": STEP_BYTES * value / (HBM_PEAK_GBS * 1e9),
+        "roofline": roofline,
+        "kernels": kern,
+        "cpu_baseline": cpu,
+        "state_after": {"mean_rho_scene0": m.mean_rho, "max_speed_scene0": m.max_speed, "nonfinite": m.nonfinite},
+    }
+    print(json.dumps(out))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -34,16 +34,16 @@ struct GridGeom {
     uint32_t ncells;
 };
 
-// Cell edge h*(1+2^-10): with float32 rounding of the hash, |xi-xj| < h still
-// lands in adjacent cells.  Grown by 2^(1/4) until each axis has <= 2048
-// cells and the grid has <= max(65536, 8n) cells.
+// Cell edge h*(1+2^-10)/kSub: with float32 rounding of the hash, |xi-xj| < h
+// still lands within kSub cells on every axis.  Grown by 2^(1/4) until each
+// axis has <= 2048 cells and the grid has <= max(2^19, 16n) cells.
 bool grid_geometry(float h, const sph_boundary_t& b, uint64_t n, GridGeom& g)
 {
     if (!(h > 0.0f) || !std::isfinite(h)) return false;
     for (int a = 0; a < 3; ++a)
         if (!std::isfinite(b.lower[a]) || !std::isfinite(b.upper[a])) return false;
-    double cs = (double)h * (1.0 + 1.0 / 1024.0);
-    const uint64_t cap = std::max<uint64_t>(65536u, 8u * n);
+    double cs = (double)h * (1.0 + 1.0 / 1024.0) / kSub;
+    const uint64_t cap = std::max<uint64_t>(524288u, 16u * n);
     for (;;) {
         uint64_t total = 1;
         bool ok = true;

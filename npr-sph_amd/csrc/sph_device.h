@@ -13,6 +13,12 @@ namespace nprsph {
 // wave64s).  Tiles never straddle scenes.
 constexpr int kTile = 256;
 
+// Cells per smoothing length: the cell edge is h*(1+2^-10)/kSub, so every
+// neighbour within h lies within kSub cells on each axis and the stencil is
+// (2*kSub+1)^3 cells, visited as (2*kSub+1)^2 rows of 2*kSub+1 x-adjacent
+// cells that are contiguous in the cell-sorted order.
+constexpr int kSub = 2;
+
 // Per-scene parameter block: the device copy of ConstantsUniform (binding 1),
 // BoundaryUniform (binding 2) and the shader-baked constants, plus the scene's
 // grid geometry and particle range.  Kernels read it with scalar loads, so a
@@ -54,14 +60,15 @@ struct alignas(16) Tile {
 // Float32 with no contraction, so the CPU build reproduces it bit for bit.
 // Returns false when any component is NaN (such particles go to the scene's
 // NaN cell, which is in no stencil: every comparison with NaN is false in the
-// reference as well).  Clamping keeps |xi-xj| < h => |ci-cj| <= 1.
-__device__ __forceinline__ bool cell_of(const SceneDev& sc, float x, float y, float z, int& cx, int& cy,
-                                        int& cz)
+// reference as well).  Clamping keeps |xi-xj| < h => |ci-cj| <= kSub.
+// f* receive the unclamped fractional cell coordinates.
+__device__ __forceinline__ bool cell_coords(const SceneDev& sc, float x, float y, float z, float& fx, float& fy,
+                                            float& fz, int& cx, int& cy, int& cz)
 {
 #pragma clang fp contract(off)
-    const float fx = (x - sc.lo[0]) * sc.inv_cs;
-    const float fy = (y - sc.lo[1]) * sc.inv_cs;
-    const float fz = (z - sc.lo[2]) * sc.inv_cs;
+    fx = (x - sc.lo[0]) * sc.inv_cs;
+    fy = (y - sc.lo[1]) * sc.inv_cs;
+    fz = (z - sc.lo[2]) * sc.inv_cs;
     if (fx != fx || fy != fy || fz != fz) return false;
     cx = fx < 0.0f ? 0 : (fx >= (float)sc.dims[0] ? sc.dims[0] - 1 : (int)fx);
     cy = fy < 0.0f ? 0 : (fy >= (float)sc.dims[1] ? sc.dims[1] - 1 : (int)fy);
@@ -72,7 +79,8 @@ __device__ __forceinline__ bool cell_of(const SceneDev& sc, float x, float y, fl
 __device__ __forceinline__ uint32_t global_key(const SceneDev& sc, float x, float y, float z)
 {
     int cx, cy, cz;
-    if (!cell_of(sc, x, y, z, cx, cy, cz)) return sc.cell_base + sc.ncells;
+    float fx, fy, fz;
+    if (!cell_coords(sc, x, y, z, fx, fy, fz, cx, cy, cz)) return sc.cell_base + sc.ncells;
     return sc.cell_base + ((uint32_t)cz * (uint32_t)sc.dims[1] + (uint32_t)cy) * (uint32_t)sc.dims[0] + (uint32_t)cx;
 }
 

@@ -22,6 +22,21 @@ constexpr int kWave = 64;
 
 __device__ __forceinline__ int lane_id() { return (int)__lane_id(); }
 
+// Workgroups are dealt to the 8 XCDs round-robin (workgroup b runs on XCD
+// b % 8), and each XCD has its own L2.  The gathers map workgroup b to tile
+// (b % 8) * per + b / 8, so every XCD walks one contiguous eighth of the
+// cell-sorted slots and its L2 holds that region's neighbours.  The launch
+// has 8*per workgroups; the ones past ntiles exit.
+constexpr int kXcds = 8;
+
+__host__ __device__ __forceinline__ uint32_t xcd_per(uint32_t ntiles) { return (ntiles + kXcds - 1) / kXcds; }
+
+__device__ __forceinline__ uint32_t xcd_tile(uint32_t ntiles)
+{
+    const uint32_t per = xcd_per(ntiles);
+    return (blockIdx.x % kXcds) * per + blockIdx.x / kXcds;
+}
+
 __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v)
 {
     const int lane = lane_id();
@@ -255,22 +270,29 @@ __global__ __launch_bounds__(kTile) void k_scatter(const Tile* __restrict__ tile
 
 // Final slot = cell start + number of cell members with a smaller AoS index,
 // i.e. the order of a stable counting sort over AoS order, independent of the
-// atomic arrival order.  Gathers pos/vel from B into A.
-__global__ __launch_bounds__(kTile) void k_rank_reorder(const Tile* __restrict__ tiles, const uint32_t* __restrict__ start,
+// atomic arrival order.  Gathers pos/vel from B into A.  Members of the NaN
+// cell interact with nothing, so they keep their arrival order (their slots
+// are the only ones whose order is unspecified).
+__global__ __launch_bounds__(kTile) void k_rank_reorder(const Tile* __restrict__ tiles, const SceneDev* __restrict__ scenes,
+                                                        const uint32_t* __restrict__ start,
                                                         const uint32_t* __restrict__ torig, const uint32_t* __restrict__ tsrc,
                                                         const uint32_t* __restrict__ tkey, const float4* __restrict__ PB,
                                                         const float4* __restrict__ VB, uint32_t* __restrict__ perm,
                                                         float4* __restrict__ PA, float4* __restrict__ VA)
 {
     const Tile t = tiles[blockIdx.x];
+    const SceneDev& sc = scenes[t.scene];
     const uint32_t q = t.begin + threadIdx.x;
     if (q >= t.end) return;
     const uint32_t k = tkey[q];
-    const uint32_t b = start[k], e = start[k + 1];
     const uint32_t o = torig[q];
-    uint32_t rank = 0;
-    for (uint32_t j = b; j < e; ++j) rank += (torig[j] < o) ? 1u : 0u;
-    const uint32_t dst = b + rank;
+    uint32_t dst = q;
+    if (k != sc.cell_base + sc.ncells) {
+        const uint32_t b = start[k], e = start[k + 1];
+        uint32_t rank = 0;
+        for (uint32_t j = b; j < e; ++j) rank += (torig[j] < o) ? 1u : 0u;
+        dst = b + rank;
+    }
     const uint32_t src = tsrc[q];
     perm[dst] = o;
     PA[dst] = PB[src];
@@ -279,15 +301,71 @@ __global__ __launch_bounds__(kTile) void k_rank_reorder(const Tile* __restrict__
 
 // ---------------------------------------------------------------- gathers ---
 
+// Visits the stencil rows of a particle at fractional cell coordinates
+// (fx,fy,fz) in cell (cx,cy,cz): rows (y,z) within kSub cells, each a slot
+// range [b,e) covering x-adjacent cells.  With PRUNE, rows whose y/z gap to
+// the particle is >= kSub cells are skipped and the x extent is cut to
+// floor(fx -+ w), w = sqrt(kSub^2 - gap^2).  kSub cells exceed h by the
+// factor 1+2^-10, which covers the float32 rounding of the gaps and of the
+// hardware square root, so pruning only drops pairs with r >= h.  Border
+// cells also hold clamped particles beyond the box; gaps measured from the
+// unclamped coordinates stay lower bounds for them.
+// The row bounds of one z-plane are fetched together (one memory latency per
+// plane) into the caller's LDS table `rows` (2*(2*kSub+1) words per thread,
+// stride kTile).
+constexpr int kRowsPerPlane = 2 * kSub + 1;
+
+template <bool PRUNE, class Body>
+__device__ __forceinline__ void for_each_row(const SceneDev& sc, const uint32_t* __restrict__ start, float fx, float fy,
+                                             float fz, int cx, int cy, int cz, uint32_t* rows, Body&& body)
+{
+    const int dx = sc.dims[0], dy = sc.dims[1], dz = sc.dims[2];
+    const int xa = cx > kSub ? cx - kSub : 0;
+    const int xb = cx < dx - kSub ? cx + kSub : dx - 1;
+    const int z0 = cz > kSub ? cz - kSub : 0, z1 = cz < dz - kSub ? cz + kSub : dz - 1;
+    constexpr float kReach2 = (float)(kSub * kSub);
+    for (int z = z0; z <= z1; ++z) {
+        const float gz = z > cz ? (float)z - fz : (z < cz ? fz - (float)(z + 1) : 0.f);
+#pragma unroll
+        for (int k = 0; k < kRowsPerPlane; ++k) {
+            const int y = cy - kSub + k;
+            uint32_t b = 0, e = 0;
+            if (y >= 0 && y < dy) {
+                int x0 = xa, x1 = xb;
+                bool keep = true;
+                if constexpr (PRUNE) {
+                    const float gy = y > cy ? (float)y - fy : (y < cy ? fy - (float)(y + 1) : 0.f);
+                    const float rem = kReach2 - fmaf(gy, gy, gz * gz);
+                    keep = rem > 0.f;
+                    const float w = __builtin_amdgcn_sqrtf(fmaxf(rem, 0.f));
+                    x0 = (int)fminf(fmaxf(floorf(fx - w), (float)xa), (float)(xb + 1));
+                    x1 = (int)fminf(fmaxf(floorf(fx + w), (float)(xa - 1)), (float)xb);
+                }
+                if (keep) {
+                    const uint32_t row = sc.cell_base + ((uint32_t)z * (uint32_t)dy + (uint32_t)y) * (uint32_t)dx;
+                    b = start[row + (uint32_t)x0];
+                    e = start[row + (uint32_t)(x1 + 1)];
+                }
+            }
+            rows[(2 * k) * kTile] = b;
+            rows[(2 * k + 1) * kTile] = e;
+        }
+        for (int k = 0; k < kRowsPerPlane; ++k) body(rows[(2 * k) * kTile], rows[(2 * k + 1) * kTile]);
+    }
+}
+
 // rho_pres_comp.glsl:35-58.  rho_i = m*315/(64 pi h^9) * sum_{r<h} (h^2-r^2)^3
-// over the 27-cell stencil (self included, :46); p_i = max(k(rho_i-rho0), 0).
-// The three x-adjacent cells of a stencil row are one contiguous slot range.
+// over the stencil (self included, :46); p_i = max(k(rho_i-rho0), 0).
+template <bool PRUNE>
 __global__ __launch_bounds__(kTile) void k_density(const Tile* __restrict__ tiles, const SceneDev* __restrict__ scenes,
                                                    const uint32_t* __restrict__ start, const float4* __restrict__ P,
                                                    float* __restrict__ rho_out, float2* __restrict__ D2,
-                                                   const uint32_t* __restrict__ perm, sph_particle_t* __restrict__ aos)
+                                                   const uint32_t* __restrict__ perm, sph_particle_t* __restrict__ aos,
+                                                   uint32_t ntiles)
 {
-    const Tile t = tiles[blockIdx.x];
+    const uint32_t ti = xcd_tile(ntiles);
+    if (ti >= ntiles) return;
+    const Tile t = tiles[ti];
     const SceneDev& sc = scenes[t.scene];
     const uint32_t s = t.begin + threadIdx.x;
     if (s >= t.end) return;
@@ -295,26 +373,27 @@ __global__ __launch_bounds__(kTile) void k_density(const Tile* __restrict__ tile
     const float h2 = sc.h2;
     float acc = 0.f;
     int cx, cy, cz;
-    if (cell_of(sc, xi.x, xi.y, xi.z, cx, cy, cz)) {
-        const int dx = sc.dims[0], dy = sc.dims[1], dz = sc.dims[2];
-        const int x0 = cx > 0 ? cx - 1 : 0;
-        const int x1 = cx < dx - 1 ? cx + 1 : dx - 1;
-        const int z0 = cz > 0 ? cz - 1 : 0, z1 = cz < dz - 1 ? cz + 1 : dz - 1;
-        const int y0 = cy > 0 ? cy - 1 : 0, y1 = cy < dy - 1 ? cy + 1 : dy - 1;
-        for (int z = z0; z <= z1; ++z) {
-            for (int y = y0; y <= y1; ++y) {
-                const uint32_t row = sc.cell_base + ((uint32_t)z * (uint32_t)dy + (uint32_t)y) * (uint32_t)dx;
-                const uint32_t b = start[row + (uint32_t)x0];
-                const uint32_t e = start[row + (uint32_t)x1 + 1u];
-                for (uint32_t j = b; j < e; ++j) {
-                    const float4 xj = P[j];
-                    const float ddx = xi.x - xj.x, ddy = xi.y - xj.y, ddz = xi.z - xj.z;
-                    const float r2 = fmaf(ddx, ddx, fmaf(ddy, ddy, ddz * ddz));
-                    const float w = h2 - r2;
-                    if (w > 0.f) acc = fmaf(w * w, w, acc);
-                }
+    float fx, fy, fz;
+    __shared__ uint32_t rows[2 * kRowsPerPlane * kTile];
+    // Term (h^2-r^2)^3 of one candidate; r >= h and NaN/Inf distances add nothing.
+    auto term = [&](const float4 xj) {
+        const float ddx = xi.x - xj.x, ddy = xi.y - xj.y, ddz = xi.z - xj.z;
+        const float r2 = fmaf(ddx, ddx, fmaf(ddy, ddy, ddz * ddz));
+        const float w = fmaxf(h2 - r2, 0.f);
+        return w * w * w;
+    };
+    if (cell_coords(sc, xi.x, xi.y, xi.z, fx, fy, fz, cx, cy, cz)) {
+        for_each_row<PRUNE>(sc, start, fx, fy, fz, cx, cy, cz, rows + threadIdx.x, [&](uint32_t b, uint32_t e) {
+            uint32_t j = b;
+            for (; j + 4 <= e; j += 4) {
+                const float4 x0 = P[j], x1 = P[j + 1], x2 = P[j + 2], x3 = P[j + 3];
+                acc += term(x0);
+                acc += term(x1);
+                acc += term(x2);
+                acc += term(x3);
             }
-        }
+            for (; j < e; ++j) acc += term(P[j]);
+        });
     }
     const float rho = acc * sc.poly6_m;
     const float p = fmaxf(sc.gas_const * (rho - sc.rho0), 0.f);
@@ -327,21 +406,30 @@ __global__ __launch_bounds__(kTile) void k_density(const Tile* __restrict__ tile
     }
 }
 
-// force_comp.glsl:35-66 over the 27-cell stencil, j != i (:50-53), r < h (:57):
+// Hits found while scanning candidates are queued per lane in LDS and then
+// processed by all lanes in lock step, so the pair-force arithmetic runs once
+// per neighbour instead of once per candidate of the slowest lane.
+constexpr int kQueue = 16;
+
+// force_comp.glsl:35-66 over the stencil, j != i (:50-53), r < h (:57):
 //   pressure  += m(p_i+p_j)/(2 rho_j) * 45/(pi h^6) * (h-r)^2 * delta/r
 //   viscosity += m(v_j-v_i)/rho_j * 45/(pi h^6) * (h-r), then * visc
 //   F = pressure + viscosity + rho_i * G
 // FUSE additionally runs integrate_comp.glsl:35-81 on the new force, writes the
 // AoS record (pos, vel, force, extras.xy) and the next step's B state.
-template <bool FUSE>
+template <bool FUSE, bool PRUNE>
 __global__ __launch_bounds__(kTile) void k_force(const Tile* __restrict__ tiles, const SceneDev* __restrict__ scenes,
                                                  const uint32_t* __restrict__ start, const float4* __restrict__ P,
                                                  const float4* __restrict__ V, const float* __restrict__ rho,
                                                  const float2* __restrict__ D2, const uint32_t* __restrict__ perm,
                                                  sph_particle_t* __restrict__ aos, float4* __restrict__ PN,
-                                                 float4* __restrict__ VN)
+                                                 float4* __restrict__ VN, uint32_t ntiles)
 {
-    const Tile t = tiles[blockIdx.x];
+    __shared__ uint32_t queue[kQueue][kTile];
+    __shared__ uint32_t rows[2 * kRowsPerPlane * kTile];
+    const uint32_t ti = xcd_tile(ntiles);
+    if (ti >= ntiles) return;
+    const Tile t = tiles[ti];
     const SceneDev& sc = scenes[t.scene];
     const uint32_t s = t.begin + threadIdx.x;
     if (s >= t.end) return;
@@ -352,42 +440,60 @@ __global__ __launch_bounds__(kTile) void k_force(const Tile* __restrict__ tiles,
     const float h = sc.h, h2 = sc.h2;
     float ax = 0.f, ay = 0.f, az = 0.f;  // pressure sum
     float bx = 0.f, by = 0.f, bz = 0.f;  // viscosity sum
-    int cx, cy, cz;
-    if (cell_of(sc, xi.x, xi.y, xi.z, cx, cy, cz)) {
-        const int dx = sc.dims[0], dy = sc.dims[1], dz = sc.dims[2];
-        const int x0 = cx > 0 ? cx - 1 : 0;
-        const int x1 = cx < dx - 1 ? cx + 1 : dx - 1;
-        const int z0 = cz > 0 ? cz - 1 : 0, z1 = cz < dz - 1 ? cz + 1 : dz - 1;
-        const int y0 = cy > 0 ? cy - 1 : 0, y1 = cy < dy - 1 ? cy + 1 : dy - 1;
-        for (int z = z0; z <= z1; ++z) {
-            for (int y = y0; y <= y1; ++y) {
-                const uint32_t row = sc.cell_base + ((uint32_t)z * (uint32_t)dy + (uint32_t)y) * (uint32_t)dx;
-                const uint32_t b = start[row + (uint32_t)x0];
-                const uint32_t e = start[row + (uint32_t)x1 + 1u];
-                for (uint32_t j = b; j < e; ++j) {
-                    if (j == s) continue;
-                    const float4 xj = P[j];
-                    const float ddx = xi.x - xj.x, ddy = xi.y - xj.y, ddz = xi.z - xj.z;
-                    const float r2 = fmaf(ddx, ddx, fmaf(ddy, ddy, ddz * ddz));
-                    if (r2 < h2) {
-                        const float inv_r = rsqrtf(r2);
-                        const float r = r2 * inv_r;
-                        const float q = h - r;
-                        const float2 dj = D2[j];
-                        const float4 vj = V[j];
-                        const float cp = (pi + dj.x) * dj.y * (q * q) * inv_r;
-                        ax = fmaf(cp, ddx, ax);
-                        ay = fmaf(cp, ddy, ay);
-                        az = fmaf(cp, ddz, az);
-                        const float cv = dj.y * q;
-                        bx = fmaf(cv, vj.x - vi.x, bx);
-                        by = fmaf(cv, vj.y - vi.y, by);
-                        bz = fmaf(cv, vj.z - vi.z, bz);
-                    }
-                }
+    int cnt = 0;
+    auto flush = [&]() {
+        for (int k = 0; __any(k < cnt); ++k) {
+            if (k < cnt) {
+                const uint32_t j = queue[k][threadIdx.x];
+                const float4 xj = P[j];
+                const float ddx = xi.x - xj.x, ddy = xi.y - xj.y, ddz = xi.z - xj.z;
+                const float r2 = fmaf(ddx, ddx, fmaf(ddy, ddy, ddz * ddz));
+                const float inv_r = rsqrtf(r2);  // r == 0 (coincident pair) gives NaN, as normalize(0)
+                const float r = r2 * inv_r;
+                const float q = h - r;
+                const float2 dj = D2[j];
+                const float4 vj = V[j];
+                const float cp = (pi + dj.x) * dj.y * (q * q) * inv_r;
+                ax = fmaf(cp, ddx, ax);
+                ay = fmaf(cp, ddy, ay);
+                az = fmaf(cp, ddz, az);
+                const float cv = dj.y * q;
+                bx = fmaf(cv, vj.x - vi.x, bx);
+                by = fmaf(cv, vj.y - vi.y, by);
+                bz = fmaf(cv, vj.z - vi.z, bz);
             }
         }
+        cnt = 0;
+    };
+    auto consider = [&](const float4 xj, uint32_t j) {
+        const float ddx = xi.x - xj.x, ddy = xi.y - xj.y, ddz = xi.z - xj.z;
+        const float r2 = fmaf(ddx, ddx, fmaf(ddy, ddy, ddz * ddz));
+        if (r2 < h2 && j != s) {
+            queue[cnt][threadIdx.x] = j;
+            ++cnt;
+        }
+    };
+    int cx, cy, cz;
+    float fx, fy, fz;
+    if (cell_coords(sc, xi.x, xi.y, xi.z, fx, fy, fz, cx, cy, cz)) {
+        for_each_row<PRUNE>(sc, start, fx, fy, fz, cx, cy, cz, rows + threadIdx.x, [&](uint32_t b, uint32_t e) {
+            uint32_t j = b;
+            for (; j + 4 <= e; j += 4) {
+                const float4 x0 = P[j], x1 = P[j + 1], x2 = P[j + 2], x3 = P[j + 3];
+                if (__any(cnt > kQueue - 4)) flush();
+                consider(x0, j);
+                consider(x1, j + 1);
+                consider(x2, j + 2);
+                consider(x3, j + 3);
+            }
+            for (; j < e; ++j) {
+                const float4 xj = P[j];
+                if (__any(cnt == kQueue)) flush();
+                consider(xj, j);
+            }
+        });
     }
+    flush();
     float f[3];
     f[0] = fmaf(sc.pres_m, ax, fmaf(sc.visc_m, bx, rhoi * sc.grav[0]));
     f[1] = fmaf(sc.pres_m, ay, fmaf(sc.visc_m, by, rhoi * sc.grav[1]));
@@ -483,6 +589,11 @@ __global__ __launch_bounds__(kTile) void k_metrics(const Tile* __restrict__ tile
 
 inline unsigned blocks_for(uint32_t n) { return (n + 255u) / 256u; }
 
+#ifndef NPRSPH_PRUNE
+#define NPRSPH_PRUNE 1
+#endif
+constexpr bool kPrune = NPRSPH_PRUNE != 0;
+
 }  // namespace
 
 // --------------------------------------------------------------- launchers ---
@@ -563,26 +674,29 @@ hipError_t launch_scatter(hipStream_t s, const DeviceState& d)
 
 hipError_t launch_rank_reorder(hipStream_t s, const DeviceState& d)
 {
-    hipLaunchKernelGGL(k_rank_reorder, dim3(d.ntiles), dim3(kTile), 0, s, d.tiles, d.start, d.torig, d.tsrc, d.tkey,
-                       d.PB, d.VB, d.perm, d.PA, d.VA);
+    hipLaunchKernelGGL(k_rank_reorder, dim3(d.ntiles), dim3(kTile), 0, s, d.tiles, d.scenes, d.start, d.torig, d.tsrc,
+                       d.tkey, d.PB, d.VB, d.perm, d.PA, d.VA);
     return hipGetLastError();
 }
 
 hipError_t launch_density(hipStream_t s, const DeviceState& d, bool write_aos)
 {
-    hipLaunchKernelGGL(k_density, dim3(d.ntiles), dim3(kTile), 0, s, d.tiles, d.scenes, d.start, d.PA, d.rho, d.D2,
-                       d.perm, write_aos ? d.aos : nullptr);
+    const uint32_t nt = (uint32_t)d.ntiles;
+    hipLaunchKernelGGL(k_density<kPrune>, dim3(kXcds * xcd_per(nt)), dim3(kTile), 0, s, d.tiles, d.scenes, d.start,
+                       d.PA, d.rho, d.D2, d.perm, write_aos ? d.aos : nullptr, nt);
     return hipGetLastError();
 }
 
 hipError_t launch_force(hipStream_t s, const DeviceState& d, bool fuse_integrate)
 {
+    const uint32_t nt = (uint32_t)d.ntiles;
+    const dim3 grid(kXcds * xcd_per(nt));
     if (fuse_integrate)
-        hipLaunchKernelGGL(k_force<true>, dim3(d.ntiles), dim3(kTile), 0, s, d.tiles, d.scenes, d.start, d.PA, d.VA,
-                           d.rho, d.D2, d.perm, d.aos, d.PB, d.VB);
+        hipLaunchKernelGGL((k_force<true, kPrune>), grid, dim3(kTile), 0, s, d.tiles, d.scenes, d.start, d.PA, d.VA,
+                           d.rho, d.D2, d.perm, d.aos, d.PB, d.VB, nt);
     else
-        hipLaunchKernelGGL(k_force<false>, dim3(d.ntiles), dim3(kTile), 0, s, d.tiles, d.scenes, d.start, d.PA, d.VA,
-                           d.rho, d.D2, d.perm, d.aos, d.PB, d.VB);
+        hipLaunchKernelGGL((k_force<false, kPrune>), grid, dim3(kTile), 0, s, d.tiles, d.scenes, d.start, d.PA, d.VA,
+                           d.rho, d.D2, d.perm, d.aos, d.PB, d.VB, nt);
     return hipGetLastError();
 }
 

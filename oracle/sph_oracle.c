@@ -59,17 +59,19 @@ float orc_smoothing_length(const orc_constants* c, const orc_fixed* f)
 
 /* ---------------------------------------------------------------- grid --- */
 
-/* Cell edge = h*(1+2^-10) so that |xi-xj| < h always lands in adjacent cells
- * despite float32 rounding of the hash; grown by 2^(1/4) steps until every
- * axis has <= 2048 cells and the grid has <= max(65536, 8n) cells. */
+/* Cell edge = h*(1+2^-10)/ORC_SUB: a particle within h of another is at most
+ * ORC_SUB cells away on every axis despite float32 rounding of the hash.  The
+ * edge grows by 2^(1/4) steps until every axis has <= 2048 cells and the grid
+ * has <= max(2^19, 16n) cells. */
+#define ORC_SUB 2
 int orc_grid_geometry(float h, const orc_boundary* b, uint64_t n, orc_grid* g)
 {
     if (!(h > 0.0f) || !isfinite(h)) return -1;
     for (int a = 0; a < 3; ++a)
         if (!isfinite(b->lower[a]) || !isfinite(b->upper[a])) return -1;
-    double cs = (double)h * (1.0 + 1.0 / 1024.0);
-    uint64_t cap = 8u * n;
-    if (cap < 65536u) cap = 65536u;
+    double cs = (double)h * (1.0 + 1.0 / 1024.0) / ORC_SUB;
+    uint64_t cap = 16u * n;
+    if (cap < 524288u) cap = 524288u;
     for (;;) {
         uint64_t total = 1;
         int ok = 1;
@@ -161,7 +163,7 @@ static void orc_cells_free(orc_cells* cl)
     free(cl->perm);
 }
 
-/* Visit the 3x3x3 cell stencil around particle i's cell.  The NaN cell has no
+/* Visit the (2*ORC_SUB+1)^3 cell stencil around particle i's cell.  The NaN cell has no
  * neighbours, as in the reference, where every comparison with NaN is false. */
 #define ORC_FOR_NEIGHBOURS(cl, i, J, ...)                                                    \
     do {                                                                                     \
@@ -171,12 +173,12 @@ static void orc_cells_free(orc_cells* cl)
             int32_t cx_ = (int32_t)(key_ % (uint32_t)dx_);                                   \
             int32_t cy_ = (int32_t)((key_ / (uint32_t)dx_) % (uint32_t)dy_);                 \
             int32_t cz_ = (int32_t)(key_ / ((uint32_t)dx_ * (uint32_t)dy_));                 \
-            for (int32_t z_ = cz_ - 1; z_ <= cz_ + 1; ++z_) {                                \
+            for (int32_t z_ = cz_ - ORC_SUB; z_ <= cz_ + ORC_SUB; ++z_) {                    \
                 if (z_ < 0 || z_ >= dz_) continue;                                           \
-                for (int32_t y_ = cy_ - 1; y_ <= cy_ + 1; ++y_) {                            \
+                for (int32_t y_ = cy_ - ORC_SUB; y_ <= cy_ + ORC_SUB; ++y_) {                \
                     if (y_ < 0 || y_ >= dy_) continue;                                       \
-                    int32_t x0_ = cx_ > 0 ? cx_ - 1 : 0;                                     \
-                    int32_t x1_ = cx_ < dx_ - 1 ? cx_ + 1 : dx_ - 1;                         \
+                    int32_t x0_ = cx_ > ORC_SUB ? cx_ - ORC_SUB : 0;                         \
+                    int32_t x1_ = cx_ < dx_ - ORC_SUB ? cx_ + ORC_SUB : dx_ - 1;             \
                     uint32_t row_ = ((uint32_t)z_ * (uint32_t)dy_ + (uint32_t)y_) * (uint32_t)dx_; \
                     for (uint32_t s_ = (cl)->start[row_ + x0_]; s_ < (cl)->start[row_ + x1_ + 1]; ++s_) { \
                         uint64_t J = (cl)->perm[s_];                                         \
@@ -216,8 +218,8 @@ void orc_density_bf(uint64_t n, const double* x, const orc_constants* c, const o
     }
 }
 
-/* Same sums restricted to the 27-cell stencil of a cell list built from the
- * float32 image of x; the cell edge is >= h, so the neighbour set is exact. */
+/* Same sums restricted to the cell stencil of a cell list built from the
+ * float32 image of x; ORC_SUB cells span >= h, so the neighbour set is exact. */
 int orc_density_grid(uint64_t n, const double* x, const orc_constants* c, const orc_fixed* f,
                      const orc_boundary* b, double* rho, double* p)
 {

@@ -1,0 +1,71 @@
+"""The C ABI library loads and exports what include/sph.h declares (no GPU calls).
+
+Also pins the ABI structs to the reference's byte layout: the 64-byte
+std430 Particle (Main.cpp:93-99), the 16-byte ConstantsUniform
+(Main.cpp:110-116) and the 32-byte BoundaryUniform (Main.cpp:118-122).
+"""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "sph.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(sph_[a-z_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol(pkg, gpu_lib):
+    names = declared_functions()
+    assert len(names) >= 25
+    for n in names:
+        assert hasattr(gpu_lib, n), f"libnprsph.so lacks {n}"
+    assert sorted(pkg.sph.EXPORTS) == names
+
+
+def test_no_oracle_symbols_in_product(pkg):
+    """The product library never links the checker."""
+    out = os.popen(f"nm -D --defined-only {pkg.sph.library_path()}").read()
+    assert "orc_" not in out
+    assert "sph_rho_pres" in out
+
+
+def test_struct_layouts(pkg):
+    s = pkg.sph
+    assert s.PARTICLE_DTYPE.itemsize == 64
+    assert C.sizeof(s.Constants) == 16
+    assert C.sizeof(s.Boundary) == 32
+    assert C.sizeof(s.Fixed) == 28
+    assert C.sizeof(s.Lattice) == 48
+    assert C.sizeof(s.GridInfo) == 56
+    assert C.sizeof(s.Metrics) == 56
+    assert s.PARTICLE_DTYPE.fields["pos"][1] == 0 and s.PARTICLE_DTYPE.fields["extras"][1] == 48
+
+
+def test_defaults_are_the_references(pkg, gpu_lib):
+    c, b, f = pkg.sph.defaults()
+    assert (c.mass, c.smoothing_coeff, c.visc, c.resting_rho) == (np.float32(0.02), 4.0, 3000.0, 1000.0)
+    assert list(b.upper) == [0.5, 1.0, 0.5, 1.0]
+    assert list(b.lower) == [np.float32(-0.1), np.float32(-0.35), np.float32(-0.1), 1.0]
+    assert f.particle_radius == np.float32(0.005) and f.gas_const == 2000.0
+    assert list(f.gravity) == [0.0, np.float32(-9806.65), 0.0]
+    assert f.damping == np.float32(0.3) and f.dt == np.float32(1.0) / np.float32(10000.0)
+
+
+def test_null_context_calls_fail_cleanly(gpu_lib):
+    """Every entry point rejects a NULL context without touching the GPU."""
+    L = gpu_lib
+    assert L.sph_rho_pres(None) == -1
+    assert L.sph_force(None) == -1
+    assert L.sph_integrate(None) == -1
+    assert L.sph_step(None, 1) == -1
+    assert L.sph_size(None) == 0
+    assert L.sph_last_error(None) == b"null context"
+    vp = C.c_void_p()
+    assert L.sph_create_batch(C.byref(vp), 0, None, None, None, None, None) == -1
+    assert b"scene" in L.sph_create_error()

@@ -39,8 +39,13 @@ def check_force(orc, state, after, ob, grid=False, c=None, f=None):
     frc, scale = orc.forces(orc.pos64(state), orc.vel64(state), state["extras"][:, 0].astype(np.float64),
                             state["extras"][:, 1].astype(np.float64), c=c, f=f, b=ob, grid=grid)
     g = after["force"][:, :3].astype(np.float64)
-    err = np.sqrt(((g - frc) ** 2).sum(axis=1))
-    assert np.all(err <= FORCE_TOL * scale + 1e-30), f"force rel err {np.max(err / scale)}"
+    # Coincident pairs give NaN through normalize(0) in the reference; the GPU
+    # must produce non-finite forces for exactly those particles.
+    bad = ~np.isfinite(frc).all(axis=1)
+    assert np.array_equal(bad, ~np.isfinite(g).all(axis=1)), "non-finite force pattern differs"
+    err = np.sqrt(((g[~bad] - frc[~bad]) ** 2).sum(axis=1))
+    sc = scale[~bad]
+    assert np.all(err <= FORCE_TOL * sc + 1e-30), f"force rel err {np.max(err / np.maximum(sc, 1e-300))}"
 
 
 def check_integrate(orc, state, after, ob, f=None):
@@ -55,6 +60,9 @@ def check_integrate(orc, state, after, ob, f=None):
     strict = np.ones_like(gx, dtype=bool)
     for a in range(3):
         strict[:, a] = (near & (1 << a)) == 0
+    bad = ~np.isfinite(x) | ~np.isfinite(v)
+    assert np.array_equal(bad, ~np.isfinite(gx) | ~np.isfinite(gv)), "non-finite state pattern differs"
+    strict &= ~bad
     assert np.all(np.abs(gv - v)[strict] <= 4 * F32_EPS * vscale[strict] + 1e-30), "velocity mismatch"
     assert np.all(np.abs(gx - x)[strict] <= 4 * F32_EPS * xscale[strict] + 1e-30), "position mismatch"
     # Near a wall either branch is acceptable, but the result must be one of them.
@@ -63,7 +71,19 @@ def check_integrate(orc, state, after, ob, f=None):
         hi = np.array(ob.upper[:3], dtype=np.float64)
         on_wall = np.isclose(gx, lo[None, :], atol=0, rtol=0) | np.isclose(gx, hi[None, :], atol=0, rtol=0)
         ok = np.abs(gx - x) <= 1e-5 + 4 * F32_EPS * xscale
-        assert np.all((ok | on_wall)[~strict])
+        assert np.all((ok | on_wall | bad)[~strict])
+
+
+def assert_same_grid(start, perm, cstart, cperm, void_cells):
+    """Cell starts equal; slot order equal except inside NaN cells, whose
+    members interact with nothing and may sit in any order."""
+    assert np.array_equal(start, cstart)
+    mask = np.ones(len(perm), dtype=bool)
+    for c in void_cells:
+        b, e = int(start[c]), int(start[c + 1])
+        assert np.array_equal(np.sort(perm[b:e]), np.sort(cperm[b:e]))
+        mask[b:e] = False
+    assert np.array_equal(perm[mask], cperm[mask])
 
 
 def test_passes_dam10k_vs_bruteforce(pkg, orc, gpu_lib):
@@ -128,8 +148,7 @@ def test_grid_bit_exact(pkg, orc, gpu_lib):
             assert tuple(gi.dims) == tuple(g.dims) and gi.ncells == g.ncells
             assert np.float32(gi.inv_cell) == np.float32(g.inv_cs)
             keys, cstart, cperm = orc.grid_build(st["pos"][:, :3], g)
-            assert np.array_equal(start, cstart)
-            assert np.array_equal(perm, cperm)
+            assert_same_grid(start, perm, cstart, cperm, [g.ncells])
 
 
 def test_fused_step_equals_passes(pkg, orc, gpu_lib):
@@ -163,7 +182,7 @@ def test_dam1m_one_step_vs_grid_oracle(pkg, orc, gpu_lib):
     ob = orc.make_boundary(sc.lower, sc.upper)
     with pkg.Simulation(sc.count, boundaries=b) as sim:
         sim.init_lattice(pkg.sph.make_lattice(**sc.lattice_args()))
-        sim.step(20)
+        sim.step(300)  # collapsed, wall-loaded state with NaN (coincident-corner) particles
         st = sim.download()
         sim.rho_pres()
         s1 = sim.download()
@@ -177,4 +196,4 @@ def test_dam1m_one_step_vs_grid_oracle(pkg, orc, gpu_lib):
         start, perm = sim.download_grid()
         g = orc.grid_geometry(orc.smoothing_length(), ob, sc.count)
         _, cstart, cperm = orc.grid_build(s2["pos"][:, :3], g)
-        assert np.array_equal(start, cstart) and np.array_equal(perm, cperm)
+        assert_same_grid(start, perm, cstart, cperm, [g.ncells])

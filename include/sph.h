@@ -99,13 +99,26 @@ typedef struct sph_lattice {
 typedef struct sph_grid_info {
     float lo[3];          /* grid origin = boundary lower corner */
     float inv_cell;       /* 1 / cell edge (float32, used by the cell hash) */
-    double cell;          /* cell edge (>= smoothing length) */
+    double cell;          /* cell edge (>= smoothing length / cells_per_h) */
     int32_t dims[3];      /* cells per axis */
     uint32_t ncells;      /* dims[0]*dims[1]*dims[2]; cell `ncells` holds NaN positions */
     uint32_t cell_base;   /* global index of this scene's first cell */
     uint32_t first;       /* first particle of this scene in the AoS buffer */
     uint32_t count;       /* particles in this scene */
+    int32_t cells_per_h;  /* stencil reach in cells */
 } sph_grid_info_t;
+
+/* Launch and grid parameters of the density/force gathers.  Every setting
+ * gives results within the FP32 tolerance of the FP64 transcription; a fixed
+ * setting gives bit-identical results run to run, and prune/block/
+ * blocks_per_xcd never change the bits (only cells_per_h changes the
+ * neighbour summation order).  Defaults: {2, 1, 256, 0}. */
+typedef struct sph_tuning {
+    int32_t cells_per_h;     /* 1..3: cell edge = h*(1+2^-10)/cells_per_h */
+    int32_t prune;           /* 1: skip stencil rows and cells farther than h from the particle */
+    int32_t block;           /* 128 or 256 particles per gather workgroup */
+    int32_t blocks_per_xcd;  /* 0: one workgroup per tile; >0: persistent workgroups per XCD */
+} sph_tuning_t;
 
 /* Per-scene reductions over the current state (sph_metrics). */
 typedef struct sph_metrics {
@@ -192,6 +205,13 @@ int sph_download_grid(sph_ctx* ctx, uint32_t* cell_start, size_t ncell_entries, 
 
 /* Reductions over scene `scene` of the AoS state (waits). */
 int sph_metrics(sph_ctx* ctx, int scene, sph_metrics_t* out);
+
+/* Gather tuning (no reference counterpart).  sph_autotune times every
+ * supported setting for `trial_steps` steps from the current state, restores
+ * that state, applies the fastest setting and reports it. */
+int sph_get_tuning(const sph_ctx* ctx, sph_tuning_t* out);
+int sph_set_tuning(sph_ctx* ctx, const sph_tuning_t* t);
+int sph_autotune(sph_ctx* ctx, int trial_steps, sph_tuning_t* best, double* best_ms_per_step);
 
 /* Kernel timing: while enabled, passes run without graphs and every kernel is
  * bracketed by HIP events on the context's stream.  sph_kernel_times returns

@@ -34,15 +34,15 @@ struct GridGeom {
     uint32_t ncells;
 };
 
-// Cell edge h*(1+2^-10)/kSub: with float32 rounding of the hash, |xi-xj| < h
-// still lands within kSub cells on every axis.  Grown by 2^(1/4) until each
+// Cell edge h*(1+2^-10)/sub: with float32 rounding of the hash, |xi-xj| < h
+// still lands within `sub` cells on every axis.  Grown by 2^(1/4) until each
 // axis has <= 2048 cells and the grid has <= max(2^19, 16n) cells.
-bool grid_geometry(float h, const sph_boundary_t& b, uint64_t n, GridGeom& g)
+bool grid_geometry(float h, const sph_boundary_t& b, uint64_t n, int sub, GridGeom& g)
 {
-    if (!(h > 0.0f) || !std::isfinite(h)) return false;
+    if (!(h > 0.0f) || !std::isfinite(h) || sub < 1 || sub > 3) return false;
     for (int a = 0; a < 3; ++a)
         if (!std::isfinite(b.lower[a]) || !std::isfinite(b.upper[a])) return false;
-    double cs = (double)h * (1.0 + 1.0 / 1024.0) / kSub;
+    double cs = (double)h * (1.0 + 1.0 / 1024.0) / sub;
     const uint64_t cap = std::max<uint64_t>(524288u, 16u * n);
     for (;;) {
         uint64_t total = 1;
@@ -97,7 +97,9 @@ struct sph_ctx {
     std::vector<sph_fixed_t> F;
     std::vector<GridGeom> geom;
     std::vector<SceneDev> sd;
-    std::vector<Tile> tiles_h;
+    std::vector<Tile> tiles_h;      // kTile-particle tiles
+    std::vector<Tile> tiles128_h;   // 128-particle tiles (gathers)
+    std::vector<int> gtile_begin[2], gtile_end[2];
     uint32_t total_cells = 0;
     uint32_t cell_cap = 0;  // capacity of count/start (start has cell_cap+1)
     int bsum_cap = 0;
@@ -105,6 +107,8 @@ struct sph_ctx {
     DeviceState d{};
     SceneDev* d_scenes = nullptr;
     Tile* d_tiles = nullptr;
+    Tile* d_tiles128 = nullptr;
+    sph_particle_t* d_backup = nullptr;  // autotune snapshot of the AoS buffer
 
     bool b_valid = false;    // B pos/vel == AoS pos/vel through perm
     bool a_valid = false;    // A pos/vel == AoS through perm and the grid matches A
@@ -199,6 +203,19 @@ void destroy_graphs(sph_ctx* c)
     }
 }
 
+sph_tuning_t default_tuning() { return sph_tuning_t{2, 1, 256, 0}; }
+
+void select_gather_tiles(sph_ctx* c)
+{
+    if (c->d.tuning.block == 128) {
+        c->d.gtiles = c->d_tiles128;
+        c->d.ngtiles = (int)c->tiles128_h.size();
+    } else {
+        c->d.gtiles = c->d_tiles;
+        c->d.ngtiles = c->d.ntiles;
+    }
+}
+
 // (Re)compute every scene's grid and parameter block; grow the cell buffers
 // when needed and upload the parameter blocks.
 int apply_params(sph_ctx* c, bool geometry_may_change)
@@ -206,7 +223,8 @@ int apply_params(sph_ctx* c, bool geometry_may_change)
     uint32_t cells = 0;
     std::vector<GridGeom> geom(c->nscenes);
     for (int s = 0; s < c->nscenes; ++s) {
-        if (!grid_geometry(c->C[s].smoothing_coeff * c->F[s].particle_radius, c->B[s], c->count[s], geom[s]))
+        if (!grid_geometry(c->C[s].smoothing_coeff * c->F[s].particle_radius, c->B[s], c->count[s],
+                           c->d.tuning.cells_per_h, geom[s]))
             return fail(c, SPH_ERR_INVALID, "invalid grid geometry");
         const uint64_t next = (uint64_t)cells + geom[s].ncells + 1u;  // + NaN cell
         if (next > 0x7fffffffull) return fail(c, SPH_ERR_INVALID, "too many grid cells");
@@ -381,7 +399,7 @@ void free_all(sph_ctx* c)
     for (auto e : c->event_pool) (void)hipEventDestroy(e);
     void* bufs[] = {c->d.aos, c->d.PA, c->d.VA, c->d.PB, c->d.VB, c->d.rho, c->d.D2, c->d.perm, c->d.key,
                     c->d.slot, c->d.torig, c->d.tsrc, c->d.tkey, c->d.count, c->d.start, c->d.bsum, c->d.partials,
-                    c->d_scenes, c->d_tiles};
+                    c->d_scenes, c->d_tiles, c->d_tiles128, c->d_backup};
     for (void* p : bufs)
         if (p) (void)hipFree(p);
     if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
@@ -480,6 +498,8 @@ int sph_create_batch(sph_ctx** out, int nscenes, const size_t* counts, const sph
         for (uint32_t b = 0; b < c->count[s]; b += kTile)
             c->tiles_h.push_back(Tile{(uint32_t)s, off + b, off + std::min<uint32_t>(b + kTile, c->count[s]), 0u});
         c->tile_end[s] = (int)c->tiles_h.size();
+        for (uint32_t b = 0; b < c->count[s]; b += 128u)
+            c->tiles128_h.push_back(Tile{(uint32_t)s, off + b, off + std::min<uint32_t>(b + 128u, c->count[s]), 0u});
         off += c->count[s];
     }
     const uint32_t n = c->n;
@@ -492,7 +512,8 @@ int sph_create_batch(sph_ctx** out, int nscenes, const size_t* counts, const sph
               dalloc(&d.slot, n) == hipSuccess && dalloc(&d.torig, n) == hipSuccess &&
               dalloc(&d.tsrc, n) == hipSuccess && dalloc(&d.tkey, n) == hipSuccess &&
               dalloc(&d.partials, (size_t)d.ntiles * 8u) == hipSuccess &&
-              dalloc(&c->d_scenes, (size_t)nscenes) == hipSuccess && dalloc(&c->d_tiles, (size_t)d.ntiles) == hipSuccess;
+              dalloc(&c->d_scenes, (size_t)nscenes) == hipSuccess && dalloc(&c->d_tiles, (size_t)d.ntiles) == hipSuccess &&
+              dalloc(&c->d_tiles128, c->tiles128_h.size()) == hipSuccess;
     if (!ok) {
         (void)hipGetLastError();
         c->err = "device allocation failed";
@@ -500,7 +521,11 @@ int sph_create_batch(sph_ctx** out, int nscenes, const size_t* counts, const sph
     }
     d.scenes = c->d_scenes;
     d.tiles = c->d_tiles;
+    d.tuning = default_tuning();
+    select_gather_tiles(c);
     if (hipMemcpyAsync(c->d_tiles, c->tiles_h.data(), sizeof(Tile) * c->tiles_h.size(), hipMemcpyHostToDevice,
+                       c->stream) != hipSuccess ||
+        hipMemcpyAsync(c->d_tiles128, c->tiles128_h.data(), sizeof(Tile) * c->tiles128_h.size(), hipMemcpyHostToDevice,
                        c->stream) != hipSuccess) {
         (void)hipGetLastError();
         c->err = "tile upload failed";
@@ -740,6 +765,7 @@ int sph_grid_info(const sph_ctx* c, int scene, sph_grid_info_t* out)
     out->cell_base = c->sd[scene].cell_base;
     out->first = c->first[scene];
     out->count = c->count[scene];
+    out->cells_per_h = c->d.tuning.cells_per_h;
     return SPH_OK;
 }
 
@@ -785,6 +811,100 @@ int sph_metrics(sph_ctx* c, int scene, sph_metrics_t* out)
     out->com[2] = acc[4] / cnt;
     out->max_speed = acc[5];
     out->nonfinite = (uint64_t)acc[6];
+    return SPH_OK;
+}
+
+int sph_get_tuning(const sph_ctx* c, sph_tuning_t* out)
+{
+    if (!c || !out) return SPH_ERR_INVALID;
+    *out = c->d.tuning;
+    return SPH_OK;
+}
+
+int sph_set_tuning(sph_ctx* c, const sph_tuning_t* t)
+{
+    if (!c) return SPH_ERR_INVALID;
+    if (!t || !tuning_supported(*t)) return fail(c, SPH_ERR_INVALID, "unsupported tuning");
+    int rc = set_device(c);
+    if (rc != SPH_OK) return rc;
+    const sph_tuning_t old = c->d.tuning;
+    c->d.tuning = *t;
+    select_gather_tiles(c);
+    rc = apply_params(c, true);
+    if (rc != SPH_OK) {
+        c->d.tuning = old;
+        select_gather_tiles(c);
+        (void)apply_params(c, true);
+        return rc;
+    }
+    destroy_graphs(c);  // launch shapes are baked into captured graphs
+    return SPH_OK;
+}
+
+int sph_autotune(sph_ctx* c, int trial_steps, sph_tuning_t* best, double* best_ms)
+{
+    if (!c) return SPH_ERR_INVALID;
+    if (trial_steps < 1) return fail(c, SPH_ERR_INVALID, "trial_steps must be >= 1");
+    int rc = set_device(c);
+    if (rc != SPH_OK) return rc;
+    if (!c->d_backup && dalloc(&c->d_backup, c->n) != hipSuccess) {
+        (void)hipGetLastError();
+        c->d_backup = nullptr;
+        return fail(c, SPH_ERR_NOMEM, "cannot allocate the autotune snapshot");
+    }
+    SPH_HIP(c, hipMemcpyAsync(c->d_backup, c->d.aos, sizeof(sph_particle_t) * c->n, hipMemcpyDeviceToDevice, c->stream));
+    const bool was_timing = c->timing;
+    c->timing = false;
+    const sph_tuning_t start_tuning = c->d.tuning;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    SPH_HIP(c, hipEventCreate(&e0));
+    SPH_HIP(c, hipEventCreate(&e1));
+    sph_tuning_t win = start_tuning;
+    float win_ms = 1e30f;
+    const int32_t bpx_options[2] = {0, 64};
+    for (int32_t sub = 1; sub <= 3 && rc == SPH_OK; ++sub)
+        for (int32_t prune = 0; prune <= 1 && rc == SPH_OK; ++prune)
+            for (int32_t block : {128, 256})
+                for (int32_t bpx : bpx_options) {
+                    if (rc != SPH_OK) break;
+                    const sph_tuning_t t{sub, prune, block, bpx};
+                    if ((rc = sph_set_tuning(c, &t)) != SPH_OK) break;
+                    if (hipMemcpyAsync(c->d.aos, c->d_backup, sizeof(sph_particle_t) * c->n, hipMemcpyDeviceToDevice,
+                                       c->stream) != hipSuccess) {
+                        rc = hip_fail(c, hipErrorUnknown, "restore snapshot");
+                        break;
+                    }
+                    invalidate_state(c);
+                    if ((rc = sph_step(c, 1)) != SPH_OK) break;  // warm-up and graph capture
+                    if (hipEventRecord(e0, c->stream) != hipSuccess) { rc = hip_fail(c, hipErrorUnknown, "event"); break; }
+                    if ((rc = sph_step(c, trial_steps)) != SPH_OK) break;
+                    float ms = 0.f;
+                    if (hipEventRecord(e1, c->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                        hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+                        rc = hip_fail(c, hipErrorUnknown, "autotune timing");
+                        break;
+                    }
+                    if (ms < win_ms) {
+                        win_ms = ms;
+                        win = t;
+                    }
+                }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    c->timing = was_timing;
+    const int trial_rc = rc;
+    const std::string trial_err = c->err;
+    rc = sph_set_tuning(c, trial_rc == SPH_OK ? &win : &start_tuning);
+    SPH_HIP(c, hipMemcpyAsync(c->d.aos, c->d_backup, sizeof(sph_particle_t) * c->n, hipMemcpyDeviceToDevice, c->stream));
+    SPH_HIP(c, hipStreamSynchronize(c->stream));
+    invalidate_state(c);
+    if (trial_rc != SPH_OK) {
+        c->err = trial_err;
+        return trial_rc;
+    }
+    if (rc != SPH_OK) return rc;
+    if (best) *best = win;
+    if (best_ms) *best_ms = (double)win_ms / trial_steps;
     return SPH_OK;
 }
 

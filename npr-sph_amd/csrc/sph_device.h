@@ -13,12 +13,6 @@ namespace nprsph {
 // wave64s).  Tiles never straddle scenes.
 constexpr int kTile = 256;
 
-// Cells per smoothing length: the cell edge is h*(1+2^-10)/kSub, so every
-// neighbour within h lies within kSub cells on each axis and the stencil is
-// (2*kSub+1)^3 cells, visited as (2*kSub+1)^2 rows of 2*kSub+1 x-adjacent
-// cells that are contiguous in the cell-sorted order.
-constexpr int kSub = 2;
-
 // Per-scene parameter block: the device copy of ConstantsUniform (binding 1),
 // BoundaryUniform (binding 2) and the shader-baked constants, plus the scene's
 // grid geometry and particle range.  Kernels read it with scalar loads, so a
@@ -60,7 +54,8 @@ struct alignas(16) Tile {
 // Float32 with no contraction, so the CPU build reproduces it bit for bit.
 // Returns false when any component is NaN (such particles go to the scene's
 // NaN cell, which is in no stencil: every comparison with NaN is false in the
-// reference as well).  Clamping keeps |xi-xj| < h => |ci-cj| <= kSub.
+// reference as well).  With a cell edge >= h/sub, clamping keeps
+// |xi-xj| < h => |ci-cj| <= sub on every axis.
 // f* receive the unclamped fractional cell coordinates.
 __device__ __forceinline__ bool cell_coords(const SceneDev& sc, float x, float y, float z, float& fx, float& fy,
                                             float& fz, int& cx, int& cy, int& cz)

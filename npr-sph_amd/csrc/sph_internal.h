@@ -22,8 +22,11 @@ inline int scan_blocks(uint32_t ncells) { return (int)((ncells + kScanBlock - 1)
 // the next neighbour build starts.
 struct DeviceState {
     const SceneDev* scenes;
-    const Tile* tiles;
+    const Tile* tiles;   // kTile-particle tiles (grid build, streaming passes)
     int ntiles;
+    const Tile* gtiles;  // tiles of tuning.block particles (gathers)
+    int ngtiles;
+    sph_tuning_t tuning;
     uint32_t n;
     uint32_t total_cells;
     sph_particle_t* aos;
@@ -44,6 +47,8 @@ struct DeviceState {
     uint32_t* bsum;   // scan block sums
     double* partials; // reduction scratch, 8 doubles per tile
 };
+
+bool tuning_supported(const sph_tuning_t& t);
 
 // Each launcher enqueues one kernel on `s` and returns hipGetLastError().
 hipError_t launch_set_scene(hipStream_t s, SceneDev* dst, const SceneDev& value);

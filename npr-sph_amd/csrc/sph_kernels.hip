@@ -12,6 +12,8 @@
 //   k_integrate_aos integrate_comp.glsl:35-81 as a streaming pass over the AoS
 // Scenes are independent; every launch covers all of them through the tile
 // table (a tile is kTile consecutive particles of one scene).
+#include <algorithm>
+
 #include "sph_internal.h"
 
 namespace nprsph {
@@ -302,33 +304,33 @@ __global__ __launch_bounds__(kTile) void k_rank_reorder(const Tile* __restrict__
 // ---------------------------------------------------------------- gathers ---
 
 // Visits the stencil rows of a particle at fractional cell coordinates
-// (fx,fy,fz) in cell (cx,cy,cz): rows (y,z) within kSub cells, each a slot
+// (fx,fy,fz) in cell (cx,cy,cz): rows (y,z) within SUB cells, each a slot
 // range [b,e) covering x-adjacent cells.  With PRUNE, rows whose y/z gap to
-// the particle is >= kSub cells are skipped and the x extent is cut to
-// floor(fx -+ w), w = sqrt(kSub^2 - gap^2).  kSub cells exceed h by the
-// factor 1+2^-10, which covers the float32 rounding of the gaps and of the
-// hardware square root, so pruning only drops pairs with r >= h.  Border
-// cells also hold clamped particles beyond the box; gaps measured from the
-// unclamped coordinates stay lower bounds for them.
+// the particle is >= SUB cells are skipped and the x extent is cut to
+// floor(fx -+ w), w = sqrt(SUB^2 - gap^2).  SUB cells exceed h by the factor
+// 1+2^-10, which covers the float32 rounding of the gaps and of the hardware
+// square root, so pruning only drops pairs with r >= h.  Border cells also
+// hold clamped particles beyond the box: gaps measured from the unclamped
+// coordinates stay lower bounds for them, and the x window always keeps a
+// border cell that its (unbounded) extent reaches.
 // The row bounds of one z-plane are fetched together (one memory latency per
-// plane) into the caller's LDS table `rows` (2*(2*kSub+1) words per thread,
-// stride kTile).
-constexpr int kRowsPerPlane = 2 * kSub + 1;
-
-template <bool PRUNE, class Body>
+// plane) into the caller's LDS table `rows` (2*(2*SUB+1) words per thread,
+// stride BLOCK).
+template <int SUB, bool PRUNE, int BLOCK, class Body>
 __device__ __forceinline__ void for_each_row(const SceneDev& sc, const uint32_t* __restrict__ start, float fx, float fy,
                                              float fz, int cx, int cy, int cz, uint32_t* rows, Body&& body)
 {
+    constexpr int kRows = 2 * SUB + 1;
     const int dx = sc.dims[0], dy = sc.dims[1], dz = sc.dims[2];
-    const int xa = cx > kSub ? cx - kSub : 0;
-    const int xb = cx < dx - kSub ? cx + kSub : dx - 1;
-    const int z0 = cz > kSub ? cz - kSub : 0, z1 = cz < dz - kSub ? cz + kSub : dz - 1;
-    constexpr float kReach2 = (float)(kSub * kSub);
+    const int xa = cx > SUB ? cx - SUB : 0;
+    const int xb = cx < dx - SUB ? cx + SUB : dx - 1;
+    const int z0 = cz > SUB ? cz - SUB : 0, z1 = cz < dz - SUB ? cz + SUB : dz - 1;
+    constexpr float kReach2 = (float)(SUB * SUB);
     for (int z = z0; z <= z1; ++z) {
         const float gz = z > cz ? (float)z - fz : (z < cz ? fz - (float)(z + 1) : 0.f);
 #pragma unroll
-        for (int k = 0; k < kRowsPerPlane; ++k) {
-            const int y = cy - kSub + k;
+        for (int k = 0; k < kRows; ++k) {
+            const int y = cy - SUB + k;
             uint32_t b = 0, e = 0;
             if (y >= 0 && y < dy) {
                 int x0 = xa, x1 = xb;
@@ -338,8 +340,10 @@ __device__ __forceinline__ void for_each_row(const SceneDev& sc, const uint32_t*
                     const float rem = kReach2 - fmaf(gy, gy, gz * gz);
                     keep = rem > 0.f;
                     const float w = __builtin_amdgcn_sqrtf(fmaxf(rem, 0.f));
-                    x0 = (int)fminf(fmaxf(floorf(fx - w), (float)xa), (float)(xb + 1));
-                    x1 = (int)fminf(fmaxf(floorf(fx + w), (float)(xa - 1)), (float)xb);
+                    // Cells 0 and dx-1 extend to -inf/+inf, so the window ends
+                    // are clamped to them before meeting the stencil bounds.
+                    x0 = (int)fmaxf(fminf(floorf(fx - w), (float)(dx - 1)), (float)xa);
+                    x1 = (int)fminf(fmaxf(floorf(fx + w), 0.f), (float)xb);
                 }
                 if (keep) {
                     const uint32_t row = sc.cell_base + ((uint32_t)z * (uint32_t)dy + (uint32_t)y) * (uint32_t)dx;
@@ -347,63 +351,81 @@ __device__ __forceinline__ void for_each_row(const SceneDev& sc, const uint32_t*
                     e = start[row + (uint32_t)(x1 + 1)];
                 }
             }
-            rows[(2 * k) * kTile] = b;
-            rows[(2 * k + 1) * kTile] = e;
+            rows[(2 * k) * BLOCK] = b;
+            rows[(2 * k + 1) * BLOCK] = e;
         }
-        for (int k = 0; k < kRowsPerPlane; ++k) body(rows[(2 * k) * kTile], rows[(2 * k + 1) * kTile]);
+        for (int k = 0; k < kRows; ++k) body(rows[(2 * k) * BLOCK], rows[(2 * k + 1) * BLOCK]);
+    }
+}
+
+// Tiles of one gather launch: workgroup b takes the tiles (b % 8)*per + k for
+// k = b/8, b/8 + G/8, ... (G = grid size, a multiple of 8), i.e. every XCD
+// walks one contiguous eighth of the cell-sorted slots.  With G = 8*per each
+// workgroup handles one tile; smaller G gives persistent workgroups.
+template <class F>
+__device__ __forceinline__ void for_each_gather_tile(uint32_t ntiles, F&& f)
+{
+    const uint32_t per = xcd_per(ntiles);
+    const uint32_t stride = gridDim.x / kXcds;
+    const uint32_t xcd = blockIdx.x % kXcds;
+    for (uint32_t k = blockIdx.x / kXcds; k < per; k += stride) {
+        const uint32_t ti = xcd * per + k;
+        if (ti >= ntiles) break;
+        f(ti);
     }
 }
 
 // rho_pres_comp.glsl:35-58.  rho_i = m*315/(64 pi h^9) * sum_{r<h} (h^2-r^2)^3
 // over the stencil (self included, :46); p_i = max(k(rho_i-rho0), 0).
-template <bool PRUNE>
-__global__ __launch_bounds__(kTile) void k_density(const Tile* __restrict__ tiles, const SceneDev* __restrict__ scenes,
+template <int SUB, bool PRUNE, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_density(const Tile* __restrict__ tiles, const SceneDev* __restrict__ scenes,
                                                    const uint32_t* __restrict__ start, const float4* __restrict__ P,
                                                    float* __restrict__ rho_out, float2* __restrict__ D2,
                                                    const uint32_t* __restrict__ perm, sph_particle_t* __restrict__ aos,
                                                    uint32_t ntiles)
 {
-    const uint32_t ti = xcd_tile(ntiles);
-    if (ti >= ntiles) return;
-    const Tile t = tiles[ti];
-    const SceneDev& sc = scenes[t.scene];
-    const uint32_t s = t.begin + threadIdx.x;
-    if (s >= t.end) return;
-    const float4 xi = P[s];
-    const float h2 = sc.h2;
-    float acc = 0.f;
-    int cx, cy, cz;
-    float fx, fy, fz;
-    __shared__ uint32_t rows[2 * kRowsPerPlane * kTile];
-    // Term (h^2-r^2)^3 of one candidate; r >= h and NaN/Inf distances add nothing.
-    auto term = [&](const float4 xj) {
-        const float ddx = xi.x - xj.x, ddy = xi.y - xj.y, ddz = xi.z - xj.z;
-        const float r2 = fmaf(ddx, ddx, fmaf(ddy, ddy, ddz * ddz));
-        const float w = fmaxf(h2 - r2, 0.f);
-        return w * w * w;
-    };
-    if (cell_coords(sc, xi.x, xi.y, xi.z, fx, fy, fz, cx, cy, cz)) {
-        for_each_row<PRUNE>(sc, start, fx, fy, fz, cx, cy, cz, rows + threadIdx.x, [&](uint32_t b, uint32_t e) {
-            uint32_t j = b;
-            for (; j + 4 <= e; j += 4) {
-                const float4 x0 = P[j], x1 = P[j + 1], x2 = P[j + 2], x3 = P[j + 3];
-                acc += term(x0);
-                acc += term(x1);
-                acc += term(x2);
-                acc += term(x3);
-            }
-            for (; j < e; ++j) acc += term(P[j]);
-        });
-    }
-    const float rho = acc * sc.poly6_m;
-    const float p = fmaxf(sc.gas_const * (rho - sc.rho0), 0.f);
-    rho_out[s] = rho;
-    D2[s] = make_float2(p, 1.0f / rho);
-    if (aos) {
-        float* e = aos[perm[s]].extras;
-        e[0] = rho;
-        e[1] = p;
-    }
+    __shared__ uint32_t rows[2 * (2 * SUB + 1) * BLOCK];
+    for_each_gather_tile(ntiles, [&](uint32_t ti) {
+        const Tile t = tiles[ti];
+        const SceneDev& sc = scenes[t.scene];
+        const uint32_t s = t.begin + threadIdx.x;
+        if (s >= t.end) return;
+        const float4 xi = P[s];
+        const float h2 = sc.h2;
+        float acc = 0.f;
+        int cx, cy, cz;
+        float fx, fy, fz;
+        // Term (h^2-r^2)^3 of one candidate; r >= h and NaN/Inf distances add nothing.
+        auto term = [&](const float4 xj) {
+            const float ddx = xi.x - xj.x, ddy = xi.y - xj.y, ddz = xi.z - xj.z;
+            const float r2 = fmaf(ddx, ddx, fmaf(ddy, ddy, ddz * ddz));
+            const float w = fmaxf(h2 - r2, 0.f);
+            return w * w * w;
+        };
+        if (cell_coords(sc, xi.x, xi.y, xi.z, fx, fy, fz, cx, cy, cz)) {
+            for_each_row<SUB, PRUNE, BLOCK>(sc, start, fx, fy, fz, cx, cy, cz, rows + threadIdx.x,
+                                            [&](uint32_t b, uint32_t e) {
+                                                uint32_t j = b;
+                                                for (; j + 4 <= e; j += 4) {
+                                                    const float4 x0 = P[j], x1 = P[j + 1], x2 = P[j + 2], x3 = P[j + 3];
+                                                    acc += term(x0);
+                                                    acc += term(x1);
+                                                    acc += term(x2);
+                                                    acc += term(x3);
+                                                }
+                                                for (; j < e; ++j) acc += term(P[j]);
+                                            });
+        }
+        const float rho = acc * sc.poly6_m;
+        const float p = fmaxf(sc.gas_const * (rho - sc.rho0), 0.f);
+        rho_out[s] = rho;
+        D2[s] = make_float2(p, 1.0f / rho);
+        if (aos) {
+            float* e = aos[perm[s]].extras;
+            e[0] = rho;
+            e[1] = p;
+        }
+    });
 }
 
 // Hits found while scanning candidates are queued per lane in LDS and then
@@ -417,110 +439,111 @@ constexpr int kQueue = 16;
 //   F = pressure + viscosity + rho_i * G
 // FUSE additionally runs integrate_comp.glsl:35-81 on the new force, writes the
 // AoS record (pos, vel, force, extras.xy) and the next step's B state.
-template <bool FUSE, bool PRUNE>
-__global__ __launch_bounds__(kTile) void k_force(const Tile* __restrict__ tiles, const SceneDev* __restrict__ scenes,
+template <bool FUSE, int SUB, bool PRUNE, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_force(const Tile* __restrict__ tiles, const SceneDev* __restrict__ scenes,
                                                  const uint32_t* __restrict__ start, const float4* __restrict__ P,
                                                  const float4* __restrict__ V, const float* __restrict__ rho,
                                                  const float2* __restrict__ D2, const uint32_t* __restrict__ perm,
                                                  sph_particle_t* __restrict__ aos, float4* __restrict__ PN,
                                                  float4* __restrict__ VN, uint32_t ntiles)
 {
-    __shared__ uint32_t queue[kQueue][kTile];
-    __shared__ uint32_t rows[2 * kRowsPerPlane * kTile];
-    const uint32_t ti = xcd_tile(ntiles);
-    if (ti >= ntiles) return;
-    const Tile t = tiles[ti];
-    const SceneDev& sc = scenes[t.scene];
-    const uint32_t s = t.begin + threadIdx.x;
-    if (s >= t.end) return;
-    const float4 xi = P[s];
-    const float4 vi = V[s];
-    const float pi = D2[s].x;
-    const float rhoi = rho[s];
-    const float h = sc.h, h2 = sc.h2;
-    float ax = 0.f, ay = 0.f, az = 0.f;  // pressure sum
-    float bx = 0.f, by = 0.f, bz = 0.f;  // viscosity sum
-    int cnt = 0;
-    auto flush = [&]() {
-        for (int k = 0; __any(k < cnt); ++k) {
-            if (k < cnt) {
-                const uint32_t j = queue[k][threadIdx.x];
-                const float4 xj = P[j];
-                const float ddx = xi.x - xj.x, ddy = xi.y - xj.y, ddz = xi.z - xj.z;
-                const float r2 = fmaf(ddx, ddx, fmaf(ddy, ddy, ddz * ddz));
-                const float inv_r = rsqrtf(r2);  // r == 0 (coincident pair) gives NaN, as normalize(0)
-                const float r = r2 * inv_r;
-                const float q = h - r;
-                const float2 dj = D2[j];
-                const float4 vj = V[j];
-                const float cp = (pi + dj.x) * dj.y * (q * q) * inv_r;
-                ax = fmaf(cp, ddx, ax);
-                ay = fmaf(cp, ddy, ay);
-                az = fmaf(cp, ddz, az);
-                const float cv = dj.y * q;
-                bx = fmaf(cv, vj.x - vi.x, bx);
-                by = fmaf(cv, vj.y - vi.y, by);
-                bz = fmaf(cv, vj.z - vi.z, bz);
+    __shared__ uint32_t queue[kQueue][BLOCK];
+    __shared__ uint32_t rows[2 * (2 * SUB + 1) * BLOCK];
+    for_each_gather_tile(ntiles, [&](uint32_t ti) {
+        const Tile t = tiles[ti];
+        const SceneDev& sc = scenes[t.scene];
+        const uint32_t s = t.begin + threadIdx.x;
+        if (s >= t.end) return;
+        const float4 xi = P[s];
+        const float4 vi = V[s];
+        const float pi = D2[s].x;
+        const float rhoi = rho[s];
+        const float h = sc.h, h2 = sc.h2;
+        float ax = 0.f, ay = 0.f, az = 0.f;  // pressure sum
+        float bx = 0.f, by = 0.f, bz = 0.f;  // viscosity sum
+        int cnt = 0;
+        auto flush = [&]() {
+            for (int k = 0; __any(k < cnt); ++k) {
+                if (k < cnt) {
+                    const uint32_t j = queue[k][threadIdx.x];
+                    const float4 xj = P[j];
+                    const float ddx = xi.x - xj.x, ddy = xi.y - xj.y, ddz = xi.z - xj.z;
+                    const float r2 = fmaf(ddx, ddx, fmaf(ddy, ddy, ddz * ddz));
+                    const float inv_r = rsqrtf(r2);  // r == 0 (coincident pair) gives NaN, as normalize(0)
+                    const float r = r2 * inv_r;
+                    const float q = h - r;
+                    const float2 dj = D2[j];
+                    const float4 vj = V[j];
+                    const float cp = (pi + dj.x) * dj.y * (q * q) * inv_r;
+                    ax = fmaf(cp, ddx, ax);
+                    ay = fmaf(cp, ddy, ay);
+                    az = fmaf(cp, ddz, az);
+                    const float cv = dj.y * q;
+                    bx = fmaf(cv, vj.x - vi.x, bx);
+                    by = fmaf(cv, vj.y - vi.y, by);
+                    bz = fmaf(cv, vj.z - vi.z, bz);
+                }
             }
+            cnt = 0;
+        };
+        auto consider = [&](const float4 xj, uint32_t j) {
+            const float ddx = xi.x - xj.x, ddy = xi.y - xj.y, ddz = xi.z - xj.z;
+            const float r2 = fmaf(ddx, ddx, fmaf(ddy, ddy, ddz * ddz));
+            if (r2 < h2 && j != s) {
+                queue[cnt][threadIdx.x] = j;
+                ++cnt;
+            }
+        };
+        int cx, cy, cz;
+        float fx, fy, fz;
+        if (cell_coords(sc, xi.x, xi.y, xi.z, fx, fy, fz, cx, cy, cz)) {
+            for_each_row<SUB, PRUNE, BLOCK>(sc, start, fx, fy, fz, cx, cy, cz, rows + threadIdx.x,
+                                            [&](uint32_t b, uint32_t e) {
+                                                uint32_t j = b;
+                                                for (; j + 4 <= e; j += 4) {
+                                                    const float4 x0 = P[j], x1 = P[j + 1], x2 = P[j + 2], x3 = P[j + 3];
+                                                    if (__any(cnt > kQueue - 4)) flush();
+                                                    consider(x0, j);
+                                                    consider(x1, j + 1);
+                                                    consider(x2, j + 2);
+                                                    consider(x3, j + 3);
+                                                }
+                                                for (; j < e; ++j) {
+                                                    const float4 xj = P[j];
+                                                    if (__any(cnt == kQueue)) flush();
+                                                    consider(xj, j);
+                                                }
+                                            });
         }
-        cnt = 0;
-    };
-    auto consider = [&](const float4 xj, uint32_t j) {
-        const float ddx = xi.x - xj.x, ddy = xi.y - xj.y, ddz = xi.z - xj.z;
-        const float r2 = fmaf(ddx, ddx, fmaf(ddy, ddy, ddz * ddz));
-        if (r2 < h2 && j != s) {
-            queue[cnt][threadIdx.x] = j;
-            ++cnt;
+        flush();
+        float f[3];
+        f[0] = fmaf(sc.pres_m, ax, fmaf(sc.visc_m, bx, rhoi * sc.grav[0]));
+        f[1] = fmaf(sc.pres_m, ay, fmaf(sc.visc_m, by, rhoi * sc.grav[1]));
+        f[2] = fmaf(sc.pres_m, az, fmaf(sc.visc_m, bz, rhoi * sc.grav[2]));
+        float* r = reinterpret_cast<float*>(aos + perm[s]);
+        if constexpr (FUSE) {
+            const float x[3] = {xi.x, xi.y, xi.z}, v[3] = {vi.x, vi.y, vi.z};
+            float nx[3], nv[3];
+            integrate_one(sc, x, v, f, rhoi, nx, nv);
+            PN[s] = make_float4(nx[0], nx[1], nx[2], 0.f);
+            VN[s] = make_float4(nv[0], nv[1], nv[2], 0.f);
+            r[0] = nx[0];
+            r[1] = nx[1];
+            r[2] = nx[2];
+            r[4] = nv[0];
+            r[5] = nv[1];
+            r[6] = nv[2];
+            r[8] = f[0];
+            r[9] = f[1];
+            r[10] = f[2];
+            r[12] = rhoi;
+            r[13] = pi;
+        } else {
+            r[8] = f[0];
+            r[9] = f[1];
+            r[10] = f[2];
         }
-    };
-    int cx, cy, cz;
-    float fx, fy, fz;
-    if (cell_coords(sc, xi.x, xi.y, xi.z, fx, fy, fz, cx, cy, cz)) {
-        for_each_row<PRUNE>(sc, start, fx, fy, fz, cx, cy, cz, rows + threadIdx.x, [&](uint32_t b, uint32_t e) {
-            uint32_t j = b;
-            for (; j + 4 <= e; j += 4) {
-                const float4 x0 = P[j], x1 = P[j + 1], x2 = P[j + 2], x3 = P[j + 3];
-                if (__any(cnt > kQueue - 4)) flush();
-                consider(x0, j);
-                consider(x1, j + 1);
-                consider(x2, j + 2);
-                consider(x3, j + 3);
-            }
-            for (; j < e; ++j) {
-                const float4 xj = P[j];
-                if (__any(cnt == kQueue)) flush();
-                consider(xj, j);
-            }
-        });
-    }
-    flush();
-    float f[3];
-    f[0] = fmaf(sc.pres_m, ax, fmaf(sc.visc_m, bx, rhoi * sc.grav[0]));
-    f[1] = fmaf(sc.pres_m, ay, fmaf(sc.visc_m, by, rhoi * sc.grav[1]));
-    f[2] = fmaf(sc.pres_m, az, fmaf(sc.visc_m, bz, rhoi * sc.grav[2]));
-    float* r = reinterpret_cast<float*>(aos + perm[s]);
-    if constexpr (FUSE) {
-        const float x[3] = {xi.x, xi.y, xi.z}, v[3] = {vi.x, vi.y, vi.z};
-        float nx[3], nv[3];
-        integrate_one(sc, x, v, f, rhoi, nx, nv);
-        PN[s] = make_float4(nx[0], nx[1], nx[2], 0.f);
-        VN[s] = make_float4(nv[0], nv[1], nv[2], 0.f);
-        r[0] = nx[0];
-        r[1] = nx[1];
-        r[2] = nx[2];
-        r[4] = nv[0];
-        r[5] = nv[1];
-        r[6] = nv[2];
-        r[8] = f[0];
-        r[9] = f[1];
-        r[10] = f[2];
-        r[12] = rhoi;
-        r[13] = pi;
-    } else {
-        r[8] = f[0];
-        r[9] = f[1];
-        r[10] = f[2];
-    }
+    });
 }
 
 // integrate_comp.glsl:35-81 as a streaming pass over the AoS buffer.
@@ -588,11 +611,6 @@ __global__ __launch_bounds__(kTile) void k_metrics(const Tile* __restrict__ tile
 }
 
 inline unsigned blocks_for(uint32_t n) { return (n + 255u) / 256u; }
-
-#ifndef NPRSPH_PRUNE
-#define NPRSPH_PRUNE 1
-#endif
-constexpr bool kPrune = NPRSPH_PRUNE != 0;
 
 }  // namespace
 
@@ -679,25 +697,66 @@ hipError_t launch_rank_reorder(hipStream_t s, const DeviceState& d)
     return hipGetLastError();
 }
 
+namespace {
+
+inline dim3 gather_grid(const DeviceState& d)
+{
+    const uint32_t per = xcd_per((uint32_t)d.ngtiles);
+    const uint32_t bpx = d.tuning.blocks_per_xcd > 0 ? std::min<uint32_t>(per, (uint32_t)d.tuning.blocks_per_xcd) : per;
+    return dim3(kXcds * std::max<uint32_t>(bpx, 1u));
+}
+
+template <int SUB, bool PRUNE, int BLOCK>
+hipError_t density_variant(hipStream_t s, const DeviceState& d, bool write_aos)
+{
+    hipLaunchKernelGGL((k_density<SUB, PRUNE, BLOCK>), gather_grid(d), dim3(BLOCK), 0, s, d.gtiles, d.scenes, d.start,
+                       d.PA, d.rho, d.D2, d.perm, write_aos ? d.aos : nullptr, (uint32_t)d.ngtiles);
+    return hipGetLastError();
+}
+
+template <int SUB, bool PRUNE, int BLOCK>
+hipError_t force_variant(hipStream_t s, const DeviceState& d, bool fuse)
+{
+    if (fuse)
+        hipLaunchKernelGGL((k_force<true, SUB, PRUNE, BLOCK>), gather_grid(d), dim3(BLOCK), 0, s, d.gtiles, d.scenes,
+                           d.start, d.PA, d.VA, d.rho, d.D2, d.perm, d.aos, d.PB, d.VB, (uint32_t)d.ngtiles);
+    else
+        hipLaunchKernelGGL((k_force<false, SUB, PRUNE, BLOCK>), gather_grid(d), dim3(BLOCK), 0, s, d.gtiles, d.scenes,
+                           d.start, d.PA, d.VA, d.rho, d.D2, d.perm, d.aos, d.PB, d.VB, (uint32_t)d.ngtiles);
+    return hipGetLastError();
+}
+
+using GatherFn = hipError_t (*)(hipStream_t, const DeviceState&, bool);
+
+#define NPRSPH_VARIANTS(fn)                                                                    \
+    {{{fn<1, false, 128>, fn<1, false, 256>}, {fn<1, true, 128>, fn<1, true, 256>}},           \
+     {{fn<2, false, 128>, fn<2, false, 256>}, {fn<2, true, 128>, fn<2, true, 256>}},           \
+     {{fn<3, false, 128>, fn<3, false, 256>}, {fn<3, true, 128>, fn<3, true, 256>}}}
+
+const GatherFn kDensity[3][2][2] = NPRSPH_VARIANTS(density_variant);
+const GatherFn kForce[3][2][2] = NPRSPH_VARIANTS(force_variant);
+
+GatherFn pick(const GatherFn (&table)[3][2][2], const sph_tuning_t& t)
+{
+    return table[t.cells_per_h - 1][t.prune ? 1 : 0][t.block == 128 ? 0 : 1];
+}
+
+}  // namespace
+
+bool tuning_supported(const sph_tuning_t& t)
+{
+    return t.cells_per_h >= 1 && t.cells_per_h <= 3 && (t.prune == 0 || t.prune == 1) &&
+           (t.block == 128 || t.block == 256) && t.blocks_per_xcd >= 0;
+}
+
 hipError_t launch_density(hipStream_t s, const DeviceState& d, bool write_aos)
 {
-    const uint32_t nt = (uint32_t)d.ntiles;
-    hipLaunchKernelGGL(k_density<kPrune>, dim3(kXcds * xcd_per(nt)), dim3(kTile), 0, s, d.tiles, d.scenes, d.start,
-                       d.PA, d.rho, d.D2, d.perm, write_aos ? d.aos : nullptr, nt);
-    return hipGetLastError();
+    return pick(kDensity, d.tuning)(s, d, write_aos);
 }
 
 hipError_t launch_force(hipStream_t s, const DeviceState& d, bool fuse_integrate)
 {
-    const uint32_t nt = (uint32_t)d.ntiles;
-    const dim3 grid(kXcds * xcd_per(nt));
-    if (fuse_integrate)
-        hipLaunchKernelGGL((k_force<true, kPrune>), grid, dim3(kTile), 0, s, d.tiles, d.scenes, d.start, d.PA, d.VA,
-                           d.rho, d.D2, d.perm, d.aos, d.PB, d.VB, nt);
-    else
-        hipLaunchKernelGGL((k_force<false, kPrune>), grid, dim3(kTile), 0, s, d.tiles, d.scenes, d.start, d.PA, d.VA,
-                           d.rho, d.D2, d.perm, d.aos, d.PB, d.VB, nt);
-    return hipGetLastError();
+    return pick(kForce, d.tuning)(s, d, fuse_integrate);
 }
 
 hipError_t launch_integrate_aos(hipStream_t s, const DeviceState& d)

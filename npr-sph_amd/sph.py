@@ -31,7 +31,7 @@ EXPORTS = [
     "sph_size", "sph_num_scenes", "sph_set_constants", "sph_set_boundary", "sph_set_fixed", "sph_get_params",
     "sph_upload", "sph_download", "sph_init_lattice", "sph_rho_pres", "sph_force", "sph_integrate", "sph_step",
     "sph_device_buffer", "sph_stream", "sph_synchronize", "sph_grid_info", "sph_total_cells", "sph_download_grid",
-    "sph_metrics", "sph_set_timing", "sph_kernel_times",
+    "sph_metrics", "sph_set_timing", "sph_kernel_times", "sph_get_tuning", "sph_set_tuning", "sph_autotune",
 ]
 
 
@@ -64,7 +64,15 @@ class Lattice(C.Structure):
 
 class GridInfo(C.Structure):
     _fields_ = [("lo", C.c_float * 3), ("inv_cell", C.c_float), ("cell", C.c_double), ("dims", C.c_int32 * 3),
-                ("ncells", C.c_uint32), ("cell_base", C.c_uint32), ("first", C.c_uint32), ("count", C.c_uint32)]
+                ("ncells", C.c_uint32), ("cell_base", C.c_uint32), ("first", C.c_uint32), ("count", C.c_uint32),
+                ("cells_per_h", C.c_int32)]
+
+
+class Tuning(C.Structure):
+    _fields_ = [("cells_per_h", C.c_int32), ("prune", C.c_int32), ("block", C.c_int32), ("blocks_per_xcd", C.c_int32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 class Metrics(C.Structure):
@@ -119,6 +127,9 @@ def load_library(path: str | None = None):
         "sph_metrics": (C.c_int, [vp, C.c_int, C.POINTER(Metrics)]),
         "sph_set_timing": (C.c_int, [vp, C.c_int]),
         "sph_kernel_times": (C.c_int, [vp, C.c_int, vp, vp, vp]),
+        "sph_get_tuning": (C.c_int, [vp, C.POINTER(Tuning)]),
+        "sph_set_tuning": (C.c_int, [vp, C.POINTER(Tuning)]),
+        "sph_autotune": (C.c_int, [vp, C.c_int, C.POINTER(Tuning), C.POINTER(C.c_double)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -282,6 +293,20 @@ class Simulation:
         m = Metrics()
         self._check(self._L.sph_metrics(self._h, scene, C.byref(m)))
         return m
+
+    def tuning(self) -> Tuning:
+        t = Tuning()
+        self._check(self._L.sph_get_tuning(self._h, C.byref(t)))
+        return t
+
+    def set_tuning(self, cells_per_h=2, prune=1, block=256, blocks_per_xcd=0):
+        t = Tuning(cells_per_h, prune, block, blocks_per_xcd)
+        self._check(self._L.sph_set_tuning(self._h, C.byref(t)))
+
+    def autotune(self, trial_steps: int = 10):
+        t, ms = Tuning(), C.c_double()
+        self._check(self._L.sph_autotune(self._h, int(trial_steps), C.byref(t), C.byref(ms)))
+        return t, ms.value
 
     def set_timing(self, on: bool):
         self._check(self._L.sph_set_timing(self._h, 1 if on else 0))

@@ -56,7 +56,7 @@ class Fixed(C.Structure):
 
 class Grid(C.Structure):
     _fields_ = [("lo", C.c_float * 3), ("inv_cs", C.c_float), ("dims", C.c_int32 * 3), ("ncells", C.c_uint32),
-                ("cs", C.c_double)]
+                ("sub", C.c_int32), ("cs", C.c_double)]
 
 
 def default_constants() -> Constants:
@@ -108,7 +108,7 @@ def lib():
         cp, bp, fp = C.POINTER(Constants), C.POINTER(Boundary), C.POINTER(Fixed)
         L.orc_smoothing_length.argtypes = [cp, fp]
         L.orc_smoothing_length.restype = C.c_float
-        L.orc_grid_geometry.argtypes = [C.c_float, bp, C.c_uint64, C.POINTER(Grid)]
+        L.orc_grid_geometry.argtypes = [C.c_float, bp, C.c_uint64, C.c_int, C.POINTER(Grid)]
         L.orc_grid_geometry.restype = C.c_int
         L.orc_cell_key.argtypes = [C.POINTER(Grid), C.c_float, C.c_float, C.c_float]
         L.orc_cell_key.restype = C.c_uint32
@@ -261,9 +261,10 @@ class State64:
 
 # ------------------------------------------------------------------- grid ---
 
-def grid_geometry(h, b=None, n: int = 0) -> Grid:
+def grid_geometry(h, b=None, n: int = 0, sub: int = 2) -> Grid:
+    """Grid geometry rule shared with the solver (sub = cells per smoothing length)."""
     g = Grid()
-    if lib().orc_grid_geometry(float(h), C.byref(_b(b)), int(n), C.byref(g)) != 0:
+    if lib().orc_grid_geometry(float(h), C.byref(_b(b)), int(n), int(sub), C.byref(g)) != 0:
         raise ValueError("invalid grid geometry")
     return g
 

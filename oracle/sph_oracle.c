@@ -45,6 +45,7 @@ typedef struct {
     float inv_cs;
     int32_t dims[3];
     uint32_t ncells; /* dims product; key `ncells` is the NaN cell */
+    int32_t sub;     /* stencil reach in cells */
     double cs;
 } orc_grid;
 
@@ -59,17 +60,17 @@ float orc_smoothing_length(const orc_constants* c, const orc_fixed* f)
 
 /* ---------------------------------------------------------------- grid --- */
 
-/* Cell edge = h*(1+2^-10)/ORC_SUB: a particle within h of another is at most
- * ORC_SUB cells away on every axis despite float32 rounding of the hash.  The
- * edge grows by 2^(1/4) steps until every axis has <= 2048 cells and the grid
- * has <= max(2^19, 16n) cells. */
-#define ORC_SUB 2
-int orc_grid_geometry(float h, const orc_boundary* b, uint64_t n, orc_grid* g)
+/* Cell edge = h*(1+2^-10)/sub (sub = cells per smoothing length, 1..3): a
+ * particle within h of another is at most `sub` cells away on every axis
+ * despite float32 rounding of the hash.  The edge grows by 2^(1/4) steps
+ * until every axis has <= 2048 cells and the grid has <= max(2^19, 16n)
+ * cells. */
+int orc_grid_geometry(float h, const orc_boundary* b, uint64_t n, int sub, orc_grid* g)
 {
-    if (!(h > 0.0f) || !isfinite(h)) return -1;
+    if (!(h > 0.0f) || !isfinite(h) || sub < 1 || sub > 3) return -1;
     for (int a = 0; a < 3; ++a)
         if (!isfinite(b->lower[a]) || !isfinite(b->upper[a])) return -1;
-    double cs = (double)h * (1.0 + 1.0 / 1024.0) / ORC_SUB;
+    double cs = (double)h * (1.0 + 1.0 / 1024.0) / sub;
     uint64_t cap = 16u * n;
     if (cap < 524288u) cap = 524288u;
     for (;;) {
@@ -87,6 +88,7 @@ int orc_grid_geometry(float h, const orc_boundary* b, uint64_t n, orc_grid* g)
         cs *= 1.189207115002721; /* 2^(1/4) */
     }
     g->cs = cs;
+    g->sub = sub;
     g->inv_cs = (float)(1.0 / cs);
     for (int a = 0; a < 3; ++a) g->lo[a] = b->lower[a];
     return 0;
@@ -143,9 +145,12 @@ typedef struct {
     uint32_t* perm;
 } orc_cells;
 
+/* The oracle's own neighbour walk uses 2 cells per h; any value gives the same
+ * neighbour sets. */
+#define ORC_SUB 2
 static int orc_cells_build(orc_cells* cl, uint64_t n, const double* x, float h, const orc_boundary* b)
 {
-    if (orc_grid_geometry(h, b, n, &cl->g) != 0) return -1;
+    if (orc_grid_geometry(h, b, n, ORC_SUB, &cl->g) != 0) return -1;
     float* xf = (float*)malloc(sizeof(float) * 3 * (n ? n : 1));
     for (uint64_t i = 0; i < 3 * n; ++i) xf[i] = (float)x[i];
     cl->keys = (uint32_t*)malloc(sizeof(uint32_t) * (n ? n : 1));

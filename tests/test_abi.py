@@ -43,6 +43,7 @@ def test_struct_layouts(pkg):
     assert C.sizeof(s.Fixed) == 28
     assert C.sizeof(s.Lattice) == 48
     assert C.sizeof(s.GridInfo) == 56
+    assert C.sizeof(s.Tuning) == 16
     assert C.sizeof(s.Metrics) == 56
     assert s.PARTICLE_DTYPE.fields["pos"][1] == 0 and s.PARTICLE_DTYPE.fields["extras"][1] == 48
 

@@ -1,0 +1,210 @@
+"""GPU tests of the solver's surface beyond the three passes: tuning variants,
+batched scenes, autotuning, parameter updates, edge cases and error paths.
+All calls go through the C ABI (libnprsph.so)."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from test_gpu_parity import assert_same_grid, check_density, check_force, check_integrate
+
+pytestmark = pytest.mark.gpu
+
+GOLD = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "kat.json")))
+
+
+def _lattice(orc, sc):
+    return orc.lattice(sc.nx, sc.ny, sc.nz, sc.spacing, sc.origin, sc.jitter, sc.seed, count=sc.count)
+
+
+def _passes(sim):
+    sim.rho_pres()
+    a = sim.download()
+    sim.force()
+    b = sim.download()
+    sim.integrate()
+    return a, b, sim.download()
+
+
+@pytest.mark.parametrize("sub", [1, 2, 3])
+def test_tuning_variants_match_oracle_and_each_other(pkg, orc, gpu_lib, sub):
+    """Every tuning gives oracle parity; prune/block/persistence never change bits."""
+    sc = pkg.scenes.dam10k()
+    p0 = _lattice(orc, sc)
+    b = pkg.sph.make_boundary(sc.lower, sc.upper)
+    ob = orc.make_boundary(sc.lower, sc.upper)
+    with pkg.Simulation(sc.count, boundaries=b) as sim:
+        sim.upload(p0)
+        sim.step(120)
+        st = sim.download()
+        results = []
+        for prune in (0, 1):
+            for block in (128, 256):
+                for bpx in (0, 3):
+                    sim.set_tuning(sub, prune, block, bpx)
+                    sim.upload(st)
+                    results.append(_passes(sim))
+        ref = results[0]
+        for r in results[1:]:
+            for x, y in zip(ref, r):
+                assert x.tobytes() == y.tobytes()
+        s1, s2, s3 = ref
+        check_density(orc, st, s1, ob, grid=False)
+        check_force(orc, s1, s2, ob, grid=False)
+        check_integrate(orc, s2, s3, ob)
+        start, perm = sim.download_grid()
+        g = orc.grid_geometry(orc.smoothing_length(), ob, sc.count, sub)
+        assert sim.grid_info(0).cells_per_h == sub and tuple(sim.grid_info(0).dims) == tuple(g.dims)
+        _, cstart, cperm = orc.grid_build(s2["pos"][:, :3], g)
+        assert_same_grid(start, perm, cstart, cperm, [g.ncells])
+
+
+def test_batch_equals_unbatched(pkg, orc, gpu_lib):
+    """Scenes advanced together in one launch per pass equal their own runs bit for bit."""
+    scenes = [s for s in pkg.scenes.batch256() if s.count < 20000][:6]
+    assert {s.layout for s in scenes} == {"dam", "column"}
+    sph = pkg.sph
+    bnds = [sph.make_boundary(s.lower, s.upper) for s in scenes]
+    with pkg.Simulation([s.count for s in scenes], boundaries=bnds) as batch:
+        for i, s in enumerate(scenes):
+            batch.init_lattice(sph.make_lattice(**s.lattice_args()), scene=i)
+        batch.step(40)
+        got = batch.download()
+        metrics = [batch.metrics(i) for i in range(len(scenes))]
+    off = 0
+    for i, s in enumerate(scenes):
+        with pkg.Simulation(s.count, boundaries=bnds[i]) as one:
+            one.init_lattice(sph.make_lattice(**s.lattice_args()))
+            one.step(40)
+            alone = one.download()
+            m = one.metrics(0)
+        assert got[off:off + s.count].tobytes() == alone.tobytes(), f"scene {i} differs"
+        assert metrics[i].mean_rho == m.mean_rho
+        off += s.count
+
+
+def test_batch_scene_parity_vs_oracle(pkg, orc, gpu_lib):
+    scenes = [s for s in pkg.scenes.batch256() if s.count < 8000][:3]
+    sph = pkg.sph
+    with pkg.Simulation([s.count for s in scenes], boundaries=[sph.make_boundary(s.lower, s.upper) for s in scenes]) as sim:
+        for i, s in enumerate(scenes):
+            sim.init_lattice(sph.make_lattice(**s.lattice_args()), scene=i)
+        sim.step(30)
+        st = sim.download()
+        sim.rho_pres()
+        s1 = sim.download()
+        sim.force()
+        s2 = sim.download()
+    off = 0
+    for s in scenes:
+        ob = orc.make_boundary(s.lower, s.upper)
+        sl = slice(off, off + s.count)
+        check_density(orc, st[sl], s1[sl], ob, grid=False)
+        check_force(orc, s1[sl], s2[sl], ob, grid=False)
+        off += s.count
+
+
+def test_autotune_keeps_state_and_picks_supported(pkg, orc, gpu_lib):
+    sc = pkg.scenes.dam10k()
+    p0 = _lattice(orc, sc)
+    with pkg.Simulation(sc.count, boundaries=pkg.sph.make_boundary(sc.lower, sc.upper)) as sim:
+        sim.upload(p0)
+        sim.step(10)
+        before = sim.download()
+        best, ms = sim.autotune(trial_steps=3)
+        assert sim.download().tobytes() == before.tobytes()
+        assert best.cells_per_h in (1, 2, 3) and best.block in (128, 256) and ms > 0
+        assert sim.tuning().as_dict() == best.as_dict()
+        sim.step(5)  # still usable
+
+
+def test_isolated_particle_closed_form(pkg, gpu_lib):
+    with pkg.Simulation(1) as sim:
+        p = np.zeros(1, dtype=pkg.sph.PARTICLE_DTYPE)
+        p["pos"][0] = (0.1, 0.2, 0.1, 1.0)
+        sim.upload(p)
+        sim.rho_pres()
+        sim.force()
+        got = sim.download()
+        assert got["extras"][0, 0] == pytest.approx(GOLD["isolated"]["rho"], rel=2e-6)
+        assert got["extras"][0, 1] == pytest.approx(GOLD["isolated"]["p"], rel=4e-6)
+        assert np.allclose(got["force"][0, :3], GOLD["isolated"]["force"], rtol=4e-6)
+        sim.integrate()
+        got = sim.download()
+        assert got["vel"][0, 1] == pytest.approx(GOLD["isolated"]["vel1"][1], rel=1e-6)
+
+
+def test_edge_cases_vs_oracle(pkg, orc, gpu_lib):
+    """Ragged count, particles outside the box, a NaN particle, a coincident pair
+    (NaN force, as normalize(0)), and a crowded cell."""
+    rng = np.random.default_rng(7)
+    n = 2053
+    p = orc.particles(n)
+    p["pos"][:, :3] = rng.uniform(-0.05, 0.05, (n, 3)).astype(np.float32)
+    p["vel"][:, :3] = rng.normal(0, 0.5, (n, 3)).astype(np.float32)
+    p["pos"][0, :3] = (0.7, 0.0, 0.0)          # beyond the upper x wall
+    p["pos"][1, :3] = (0.69, 0.0, 0.0)         # its neighbour, also outside
+    p["pos"][2, :3] = (np.nan, 0.0, 0.0)
+    p["pos"][3, :3] = p["pos"][4, :3]          # coincident pair
+    p["pos"][10:200, :3] = (rng.uniform(0, 1e-3, (190, 3)) + 0.02).astype(np.float32)  # crowded cell
+    ob = orc.default_boundary()
+    with pkg.Simulation(n) as sim:
+        sim.upload(p)
+        s1, s2, s3 = _passes(sim)
+    check_density(orc, p, s1, ob, grid=False)
+    check_force(orc, s1, s2, ob, grid=False)
+    check_integrate(orc, s2, s3, ob)
+    assert np.isnan(s2["force"][3, :3]).all() and np.isnan(s2["force"][4, :3]).all()
+    assert s1["extras"][2, 0] == 0.0 and s1["extras"][0, 0] > 0 and s1["extras"][1, 0] > GOLD["isolated"]["rho"]
+
+
+def test_set_constants_regrids(pkg, orc, gpu_lib):
+    """The reference's smoothing slider (Main.cpp:244) changes h; the grid follows."""
+    sc = pkg.scenes.dam10k()
+    p0 = _lattice(orc, sc)
+    ob = orc.make_boundary(sc.lower, sc.upper)
+    with pkg.Simulation(sc.count, boundaries=pkg.sph.make_boundary(sc.lower, sc.upper)) as sim:
+        sim.upload(p0)
+        dims0 = tuple(sim.grid_info(0).dims)
+        c = pkg.sph.Constants(0.02, 7.0, 3000.0, 1000.0)
+        sim.set_constants(c)
+        assert tuple(sim.grid_info(0).dims) != dims0
+        s1, s2, _ = _passes(sim)
+    oc = orc.Constants(0.02, 7.0, 3000.0, 1000.0)
+    check_density(orc, p0, s1, ob, grid=False, c=oc)
+    check_force(orc, s1, s2, ob, grid=False, c=oc)
+
+
+def test_error_paths_keep_context_usable(pkg, gpu_lib):
+    with pkg.Simulation(100) as sim:
+        with pytest.raises(pkg.SphError) as e:
+            sim.set_constants(pkg.sph.Constants(0.02, -1.0, 3000.0, 1000.0))
+        assert e.value.code == -1
+        with pytest.raises(pkg.SphError):
+            sim.upload(np.zeros(5, dtype=pkg.sph.PARTICLE_DTYPE), first=99)
+        with pytest.raises(pkg.SphError):
+            sim.set_tuning(cells_per_h=4)
+        with pytest.raises(pkg.SphError):
+            sim.init_lattice(pkg.sph.make_lattice(2, 2, 2, 0.01, (0, 0, 0), 0.0, 1))  # count 8 != 100
+        c, _, _ = sim.params(0)
+        assert c.smoothing_coeff == 4.0
+        sim.step(3)
+        assert sim.download().shape == (100,)
+    with pytest.raises(pkg.SphError):
+        pkg.Simulation(0)
+
+
+def test_metrics_match_numpy(pkg, orc, gpu_lib):
+    sc = pkg.scenes.dam10k()
+    with pkg.Simulation(sc.count, boundaries=pkg.sph.make_boundary(sc.lower, sc.upper)) as sim:
+        sim.init_lattice(pkg.sph.make_lattice(**sc.lattice_args()))
+        sim.step(50)
+        st = sim.download()
+        m = sim.metrics(0)
+    v = st["vel"][:, :3].astype(np.float64)
+    assert m.mean_rho == pytest.approx(st["extras"][:, 0].astype(np.float64).mean(), rel=1e-12)
+    assert m.kinetic == pytest.approx(0.5 * float(np.float32(0.02)) * (v * v).sum(), rel=1e-10)
+    assert m.max_speed == pytest.approx(np.sqrt((v * v).sum(1)).max(), rel=1e-12)
+    assert np.allclose(m.com, st["pos"][:, :3].astype(np.float64).mean(0), rtol=1e-10)
+    assert m.nonfinite == 0

@@ -144,7 +144,7 @@ def test_grid_bit_exact(pkg, orc, gpu_lib):
             sim.rho_pres()
             start, perm = sim.download_grid()
             gi = sim.grid_info(0)
-            g = orc.grid_geometry(orc.smoothing_length(), ob, sc.count)
+            g = orc.grid_geometry(orc.smoothing_length(), ob, sc.count, gi.cells_per_h)
             assert tuple(gi.dims) == tuple(g.dims) and gi.ncells == g.ncells
             assert np.float32(gi.inv_cell) == np.float32(g.inv_cs)
             keys, cstart, cperm = orc.grid_build(st["pos"][:, :3], g)
@@ -194,6 +194,6 @@ def test_dam1m_one_step_vs_grid_oracle(pkg, orc, gpu_lib):
         s3 = sim.download()
         check_integrate(orc, s2, s3, ob)
         start, perm = sim.download_grid()
-        g = orc.grid_geometry(orc.smoothing_length(), ob, sc.count)
+        g = orc.grid_geometry(orc.smoothing_length(), ob, sc.count, sim.grid_info(0).cells_per_h)
         _, cstart, cperm = orc.grid_build(s2["pos"][:, :3], g)
         assert_same_grid(start, perm, cstart, cperm, [g.ncells])

@@ -235,7 +235,7 @@ def test_grid_build_matches_numpy(orc, n):
         pos[3:6] = np.inf
         pos[6:9] = -np.inf
         pos[9, 0] = np.float32(0.5)  # exactly on the upper wall
-    g = orc.grid_geometry(orc.smoothing_length(), b, n)
+    g = orc.grid_geometry(orc.smoothing_length(), b, n, sub=1 + n % 3)
     keys, start, perm = orc.grid_build(pos, g)
     ref = _numpy_keys(pos, g)
     assert np.array_equal(keys, ref)
@@ -246,11 +246,17 @@ def test_grid_build_matches_numpy(orc, n):
 
 
 def test_grid_geometry_rule(orc):
-    """Cell edge h(1+2^-10)/2; dims cover the box; caps hold."""
+    """Cell edge h(1+2^-10)/sub; dims cover the box; caps hold."""
     h = orc.smoothing_length()
     g = orc.grid_geometry(h, orc.default_boundary(), 10000)
     assert g.cs == pytest.approx(float(h) * (1 + 2 ** -10) / 2, rel=1e-15)
     assert tuple(g.dims) == (60, 135, 60)
+    g1 = orc.grid_geometry(h, orc.default_boundary(), 10000, sub=1)
+    assert tuple(g1.dims) == (30, 68, 30) and g1.sub == 1
+    g3 = orc.grid_geometry(h, orc.default_boundary(), 10 ** 6, sub=3)
+    assert g3.cs == pytest.approx(float(h) * (1 + 2 ** -10) / 3, rel=1e-15) and g3.sub == 3
+    with pytest.raises(ValueError):
+        orc.grid_geometry(h, orc.default_boundary(), 10, sub=4)
     big = orc.make_boundary((-100, -100, -100), (100, 100, 100))
     g2 = orc.grid_geometry(h, big, 1000)
     assert max(g2.dims) <= 2048 and g2.ncells <= 2 ** 19

@@ -77,6 +77,19 @@ def cpu_baseline(pkg, scene, budget_s: float):
                       f"step, {threads} OpenMP threads ({el:.1f} s)"}
 
 
+def measured_traffic(workload, kernel, tuning):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary (calibrated by a plain copy), when it was measured for
+    this workload and tuning; otherwise None."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{workload}.json")
+    if not os.path.exists(path):
+        return None
+    rec = json.load(open(path))
+    if rec.get("kernel") != kernel or rec.get("tuning") != tuning:
+        return None
+    return rec["traffic_bytes_per_launch"]
+
+
 def copy_probe(mbytes: int, reps: int = 20):
     """Plain device copy of known size (calibrates rocprof FETCH/WRITE counters)."""
     import torch
@@ -118,12 +131,12 @@ def main(argv=None):
     scenes = _scene(pkg, args.workload)
     n = sum(s.count for s in scenes)
     sim = make_sim(pkg, scenes)
+    if args.tune not in ("auto", "default"):
+        sim.set_tuning(*[int(v) for v in args.tune.split(",")])
     sim.step(args.warmup)
     sim.synchronize()
     if args.tune == "auto":
         sim.autotune(trial_steps=args.tune_steps)
-    elif args.tune != "default":
-        sim.set_tuning(*[int(v) for v in args.tune.split(",")])
     tuning = sim.tuning().as_dict()
     sim.step(1)  # capture the step graphs of the chosen tuning outside the timed region
     sim.synchronize()
@@ -147,8 +160,9 @@ def main(argv=None):
     if algo is not None:
         achieved = algo * n / (dom_ms * 1e-3) / 1e9
         roofline = {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": achieved / HBM_PEAK_GBS, "traffic": None, "algo_bytes_per_particle": algo,
-                    "launch_ms": dom_ms}
+                    "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(args.workload, dom, tuning),
+                    "algo_bytes_per_particle": algo, "launch_ms": dom_ms,
+                    "binding_roof": "VALU issue (gathers are instruction-bound; see DESIGN.md section 5)"}
     m = sim.metrics(0)
     sim.close()
 

@@ -213,6 +213,83 @@ int sph_get_tuning(const sph_ctx* ctx, sph_tuning_t* out);
 int sph_set_tuning(sph_ctx* ctx, const sph_tuning_t* t);
 int sph_autotune(sph_ctx* ctx, int trial_steps, sph_tuning_t* best, double* best_ms_per_step);
 
+/* ---- Learned coarse step (no reference counterpart) ----------------------
+ * A pair-message MLP over every neighbour pair within h: features
+ *   (x_i-x_j)/h, (v_i-v_j)*v_scale, rho_i*rho_scale, rho_j*rho_scale,
+ *   (p_i+p_j)/(2 rho_j)*pi_scale
+ * -> 64 -> GELU -> 64 -> GELU -> 3, summed per particle: a_i = acc_scale*sum + G.
+ * One coarse step replaces 4 fine steps: v' = v + 4dt a, x' = x + 4dt v +
+ * 10 dt^2 a (semi-implicit Euler with a held fixed), then the wall clamp.
+ * Parameters: float32, SPH_MLP_PARAMS values laid out as
+ * W1[9][64], b1[64], W2[64][64], b2[64], W3[64][3], b3[3]. */
+#define SPH_MLP_PARAMS 4995
+
+typedef struct sph_mlp_norm {
+    float v_scale;
+    float rho_scale;
+    float pi_scale;
+    float acc_scale;
+} sph_mlp_norm_t;
+
+typedef struct sph_train_opts {
+    uint32_t first;    /* target particles: cell-sorted slots first, first+stride, ... */
+    uint32_t stride;
+    float lr;          /* Adam step size and moments */
+    float beta1;
+    float beta2;
+    float eps;
+    int32_t update;    /* 0: gradient only; 1: gradient and one Adam step */
+} sph_train_opts_t;
+
+typedef struct sph_train_result {
+    double loss;       /* weighted mean squared error of the summed messages */
+    uint64_t pairs;    /* neighbour pairs in the batch */
+    uint32_t targets;
+    uint32_t weighted; /* targets with weight 1 (off the walls, finite) */
+} sph_train_result_t;
+
+typedef struct sph_coarse_opts {
+    int32_t use_network;  /* 1: learned messages; 0: the SPH force held over the 4 fine steps */
+    int32_t judge_every;  /* every judge_every-th coarse step is also run as 4 plain steps (0: never) */
+    float drift_limit;    /* judge: |mean rho(coarse) - mean rho(plain)| above this keeps the plain result */
+    int32_t cooldown;     /* after a rejected step, this many coarse steps run as 4 plain steps each */
+} sph_coarse_opts_t;
+
+typedef struct sph_coarse_stats {
+    int64_t coarse_steps;  /* accepted coarse steps */
+    int64_t plain_steps;   /* fine steps run by the plain solver (fallback and cooldown) */
+    int64_t judged;
+    int64_t fallbacks;
+    double last_drift;
+    double max_drift;
+} sph_coarse_stats_t;
+
+/* Load / read the network (float32 master weights) and its input scales. */
+int sph_mlp_set(sph_ctx* ctx, const float* params, size_t n, const sph_mlp_norm_t* norm);
+int sph_mlp_get(sph_ctx* ctx, float* params, size_t n, sph_mlp_norm_t* norm);
+
+/* One training step on the GPU.  `ctx` holds the state at time t, `later` the
+ * same particles 4 fine steps later (equal particle counts and scenes).
+ * Targets are the mean accelerations (v(t+4dt)-v(t))/(4dt) - G of the chosen
+ * particles in units of acc_scale; particles on a wall or non-finite get
+ * weight 0.  Runs the density pass on ctx, enumerates the neighbour pairs,
+ * runs the forward and backward kernels and, if opts->update, one Adam step.
+ * grad_out (may be NULL) receives the SPH_MLP_PARAMS gradient. */
+int sph_mlp_train(sph_ctx* ctx, const sph_ctx* later, const sph_train_opts_t* opts, float* grad_out,
+                  sph_train_result_t* result);
+
+/* Summed messages of every particle at the current state (AoS order, 3 floats
+ * per particle, units of acc_scale).  path 0: float32 kernel; 1: BF16 MFMA
+ * kernel.  Runs the density pass first (as sph_rho_pres). */
+int sph_mlp_predict(sph_ctx* ctx, int path, float* out);
+
+/* n coarse steps (each = 4 fine steps of simulated time), with the plain
+ * solver as judge and fallback.  stats may be NULL. */
+int sph_coarse_step(sph_ctx* ctx, int n, const sph_coarse_opts_t* opts, sph_coarse_stats_t* stats);
+
+/* Copy the whole AoS state of src into dst (same particle count). */
+int sph_copy_state(sph_ctx* dst, const sph_ctx* src);
+
 /* Kernel timing: while enabled, passes run without graphs and every kernel is
  * bracketed by HIP events on the context's stream.  sph_kernel_times returns
  * the accumulated milliseconds and launch counts per kernel name. */

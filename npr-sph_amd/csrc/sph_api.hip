@@ -116,6 +116,13 @@ struct sph_ctx {
 
     hipGraphExec_t graph[2] = {nullptr, nullptr};  // 1 step, kUnroll steps
 
+    MlpWork mlp;                               // learned coarse step
+    bool mlp_loaded = false;
+    int64_t adam_step = 0;
+    sph_particle_t* d_judge_s = nullptr;       // judge: state before the coarse step
+    sph_particle_t* d_judge_p = nullptr;       // judge: state after 4 plain steps
+    float* d_pred_aos = nullptr;               // sph_mlp_predict output staging
+
     bool timing = false;
     std::vector<PendingTimer> pending;
     std::vector<hipEvent_t> event_pool;
@@ -397,13 +404,103 @@ void free_all(sph_ctx* c)
         (void)hipEventDestroy(p.e1);
     }
     for (auto e : c->event_pool) (void)hipEventDestroy(e);
+    const MlpWork& w = c->mlp;
     void* bufs[] = {c->d.aos, c->d.PA, c->d.VA, c->d.PB, c->d.VB, c->d.rho, c->d.D2, c->d.perm, c->d.key,
                     c->d.slot, c->d.torig, c->d.tsrc, c->d.tkey, c->d.count, c->d.start, c->d.bsum, c->d.partials,
-                    c->d_scenes, c->d_tiles, c->d_tiles128, c->d_backup};
+                    c->d_scenes, c->d_tiles, c->d_tiles128, c->d_backup, c->d_judge_s, c->d_judge_p, c->d_pred_aos,
+                    w.params, w.adam_m, w.adam_v, w.grad, w.cnt, w.off, w.tgt, w.wt, w.pred, w.dpred, w.lossp,
+                    w.pj, w.pt, w.X, w.Z1, w.Z2, w.dZ1, w.dZ2, w.M, w.partial};
     for (void* p : bufs)
         if (p) (void)hipFree(p);
     if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
 }
+
+// ---- learned coarse step helpers ----
+
+template <class T>
+int grow(sph_ctx* c, T** p, uint64_t& cap, uint64_t need, uint64_t per_item, const char* what)
+{
+    if (need <= cap && *p) return SPH_OK;
+    SPH_HIP(c, hipStreamSynchronize(c->stream));
+    if (*p) SPH_HIP(c, hipFree(*p));
+    *p = nullptr;
+    const uint64_t want = need + need / 4 + 1;
+    if (dalloc(p, (size_t)(want * per_item)) != hipSuccess) {
+        (void)hipGetLastError();
+        cap = 0;
+        return fail(c, SPH_ERR_NOMEM, std::string("cannot allocate ") + what);
+    }
+    cap = want;
+    return SPH_OK;
+}
+
+int ensure_mlp_base(sph_ctx* c)
+{
+    MlpWork& w = c->mlp;
+    if (w.params) return SPH_OK;
+    if (dalloc(&w.params, kMlpParams) != hipSuccess || dalloc(&w.adam_m, kMlpParams) != hipSuccess ||
+        dalloc(&w.adam_v, kMlpParams) != hipSuccess || dalloc(&w.grad, kMlpParams) != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(c, SPH_ERR_NOMEM, "cannot allocate network parameters");
+    }
+    SPH_HIP(c, hipMemsetAsync(w.adam_m, 0, sizeof(float) * kMlpParams, c->stream));
+    SPH_HIP(c, hipMemsetAsync(w.adam_v, 0, sizeof(float) * kMlpParams, c->stream));
+    return SPH_OK;
+}
+
+// Per-target buffers sized for T targets.
+int ensure_targets(sph_ctx* c, uint64_t T)
+{
+    MlpWork& w = c->mlp;
+    if (T <= w.target_cap && w.cnt) return SPH_OK;
+    uint64_t cap = w.target_cap;
+    int rc;
+    uint64_t c1 = cap, c2 = cap, c3 = cap, c4 = cap, c5 = cap, c6 = cap, c7 = cap;
+    if ((rc = grow(c, &w.cnt, c1, T, 1, "pair counts")) != SPH_OK) return rc;
+    if ((rc = grow(c, &w.off, c2, T + 1, 1, "pair offsets")) != SPH_OK) return rc;
+    if ((rc = grow(c, &w.tgt, c3, T, 3, "targets")) != SPH_OK) return rc;
+    if ((rc = grow(c, &w.wt, c4, T, 1, "target weights")) != SPH_OK) return rc;
+    if ((rc = grow(c, &w.pred, c5, T, 3, "predictions")) != SPH_OK) return rc;
+    if ((rc = grow(c, &w.dpred, c6, T, 3, "prediction gradients")) != SPH_OK) return rc;
+    if ((rc = grow(c, &w.lossp, c7, T, 1, "loss terms")) != SPH_OK) return rc;
+    w.target_cap = std::min({c1, c2 - 1, c3, c4, c5, c6, c7});
+    return SPH_OK;
+}
+
+int ensure_pairs(sph_ctx* c, uint64_t P)
+{
+    MlpWork& w = c->mlp;
+    if (P <= w.pair_cap && w.pj) return SPH_OK;
+    uint64_t cap[10];
+    for (auto& x : cap) x = w.pair_cap;
+    int rc;
+    if ((rc = grow(c, &w.pj, cap[0], P, 1, "pairs")) != SPH_OK) return rc;
+    if ((rc = grow(c, &w.pt, cap[1], P, 1, "pair targets")) != SPH_OK) return rc;
+    if ((rc = grow(c, &w.X, cap[2], P, kMlpIn, "pair features")) != SPH_OK) return rc;
+    if ((rc = grow(c, &w.Z1, cap[3], P, kMlpHidden, "layer-1 activations")) != SPH_OK) return rc;
+    if ((rc = grow(c, &w.Z2, cap[4], P, kMlpHidden, "layer-2 activations")) != SPH_OK) return rc;
+    if ((rc = grow(c, &w.dZ1, cap[5], P, kMlpHidden, "layer-1 gradients")) != SPH_OK) return rc;
+    if ((rc = grow(c, &w.dZ2, cap[6], P, kMlpHidden, "layer-2 gradients")) != SPH_OK) return rc;
+    if ((rc = grow(c, &w.M, cap[7], P, kMlpOut, "messages")) != SPH_OK) return rc;
+    w.pair_cap = *std::min_element(cap, cap + 8);
+    return SPH_OK;
+}
+
+// Density pass on the current AoS state (sph_rho_pres without the public
+// entry point), leaving A, rho/D2 and the grid valid.
+int density_pass(sph_ctx* c, bool write_aos)
+{
+    int rc;
+    if ((rc = ensure_b(c)) != SPH_OK) return rc;
+    if ((rc = enqueue_build(c)) != SPH_OK) return rc;
+    c->b_valid = false;
+    c->a_valid = true;
+    SPH_RUN(c, "density", launch_density(c->stream, c->d, write_aos));
+    c->rho_valid = true;
+    return SPH_OK;
+}
+
+int mean_rho(sph_ctx* c, double* out);
 
 }  // namespace
 
@@ -905,6 +1002,242 @@ int sph_autotune(sph_ctx* c, int trial_steps, sph_tuning_t* best, double* best_m
     if (rc != SPH_OK) return rc;
     if (best) *best = win;
     if (best_ms) *best_ms = (double)win_ms / trial_steps;
+    return SPH_OK;
+}
+
+int sph_mlp_set(sph_ctx* c, const float* params, size_t n, const sph_mlp_norm_t* norm)
+{
+    if (!c) return SPH_ERR_INVALID;
+    if (!params || n != (size_t)kMlpParams || !norm) return fail(c, SPH_ERR_INVALID, "need SPH_MLP_PARAMS weights and a norm");
+    const float nv[4] = {norm->v_scale, norm->rho_scale, norm->pi_scale, norm->acc_scale};
+    if (!finite_all(nv, 4) || !finite_all(params, (int)n)) return fail(c, SPH_ERR_INVALID, "non-finite weights or scales");
+    int rc = set_device(c);
+    if (rc != SPH_OK) return rc;
+    if ((rc = ensure_mlp_base(c)) != SPH_OK) return rc;
+    SPH_HIP(c, hipMemcpyAsync(c->mlp.params, params, sizeof(float) * n, hipMemcpyHostToDevice, c->stream));
+    SPH_HIP(c, hipMemsetAsync(c->mlp.adam_m, 0, sizeof(float) * kMlpParams, c->stream));
+    SPH_HIP(c, hipMemsetAsync(c->mlp.adam_v, 0, sizeof(float) * kMlpParams, c->stream));
+    SPH_HIP(c, hipStreamSynchronize(c->stream));
+    c->mlp.norm = MlpNorm{norm->v_scale, norm->rho_scale, norm->pi_scale, norm->acc_scale};
+    c->mlp_loaded = true;
+    c->adam_step = 0;
+    return SPH_OK;
+}
+
+int sph_mlp_get(sph_ctx* c, float* params, size_t n, sph_mlp_norm_t* norm)
+{
+    if (!c) return SPH_ERR_INVALID;
+    if (!c->mlp_loaded) return fail(c, SPH_ERR_STATE, "no network loaded");
+    int rc = set_device(c);
+    if (rc != SPH_OK) return rc;
+    if (params) {
+        if (n != (size_t)kMlpParams) return fail(c, SPH_ERR_INVALID, "need SPH_MLP_PARAMS entries");
+        SPH_HIP(c, hipMemcpyAsync(params, c->mlp.params, sizeof(float) * n, hipMemcpyDeviceToHost, c->stream));
+        SPH_HIP(c, hipStreamSynchronize(c->stream));
+    }
+    if (norm) *norm = sph_mlp_norm_t{c->mlp.norm.v_scale, c->mlp.norm.rho_scale, c->mlp.norm.pi_scale, c->mlp.norm.acc_scale};
+    return SPH_OK;
+}
+
+int sph_copy_state(sph_ctx* dst, const sph_ctx* src)
+{
+    if (!dst) return SPH_ERR_INVALID;
+    if (!src || src->n != dst->n) return fail(dst, SPH_ERR_INVALID, "source must have the same particle count");
+    int rc = set_device(dst);
+    if (rc != SPH_OK) return rc;
+    SPH_HIP(dst, hipStreamSynchronize(src->stream));
+    SPH_HIP(dst, hipMemcpyAsync(dst->d.aos, src->d.aos, sizeof(sph_particle_t) * dst->n, hipMemcpyDeviceToDevice,
+                                dst->stream));
+    invalidate_state(dst);
+    return SPH_OK;
+}
+
+int sph_mlp_train(sph_ctx* c, const sph_ctx* later, const sph_train_opts_t* o, float* grad_out, sph_train_result_t* res)
+{
+    if (!c) return SPH_ERR_INVALID;
+    if (!o || !later || later->n != c->n || later->nscenes != c->nscenes)
+        return fail(c, SPH_ERR_INVALID, "need options and a later state of the same particles");
+    if (!c->mlp_loaded) return fail(c, SPH_ERR_STATE, "no network loaded (sph_mlp_set)");
+    if (o->stride < 1 || o->first >= c->n) return fail(c, SPH_ERR_INVALID, "bad target range");
+    int rc = set_device(c);
+    if (rc != SPH_OK) return rc;
+    c->timing = false;
+    SPH_HIP(c, hipStreamSynchronize(later->stream));
+    if ((rc = density_pass(c, true)) != SPH_OK) return rc;
+    const uint32_t T = (c->n - o->first + o->stride - 1) / o->stride;
+    if ((rc = ensure_targets(c, T)) != SPH_OK) return rc;
+    MlpWork& w = c->mlp;
+    SPH_HIP(c, launch_pairs(c->stream, c->d, c->nscenes, w, o->first, o->stride, T, false));
+    std::vector<uint32_t> cnt(T);
+    SPH_HIP(c, hipMemcpyAsync(cnt.data(), w.cnt, sizeof(uint32_t) * T, hipMemcpyDeviceToHost, c->stream));
+    SPH_HIP(c, hipStreamSynchronize(c->stream));
+    std::vector<uint64_t> off(T + 1);
+    off[0] = 0;
+    for (uint32_t t = 0; t < T; ++t) off[t + 1] = off[t] + cnt[t];
+    const uint64_t P = off[T];
+    if ((rc = ensure_pairs(c, std::max<uint64_t>(P, 1))) != SPH_OK) return rc;
+    const uint32_t R = 2048;
+    const uint64_t nblocks = std::max<uint64_t>((P + R - 1) / R, 1);
+    if (nblocks > w.partial_cap || !w.partial) {
+        uint64_t cap = w.partial_cap;
+        if ((rc = grow(c, &w.partial, cap, nblocks, kMlpParams, "gradient partials")) != SPH_OK) return rc;
+        w.partial_cap = cap;
+    }
+    SPH_HIP(c, hipMemcpyAsync(w.off, off.data(), sizeof(uint64_t) * (T + 1), hipMemcpyHostToDevice, c->stream));
+    SPH_HIP(c, launch_pairs(c->stream, c->d, c->nscenes, w, o->first, o->stride, T, true));
+    SPH_HIP(c, launch_train_targets(c->stream, c->d, c->nscenes, w, later->d.aos, o->first, o->stride, T));
+    std::vector<float> wt(T);
+    SPH_HIP(c, hipMemcpyAsync(wt.data(), w.wt, sizeof(float) * T, hipMemcpyDeviceToHost, c->stream));
+    SPH_HIP(c, hipStreamSynchronize(c->stream));
+    double wsum = 0.0;
+    for (float x : wt) wsum += x;
+    const float inv_wsum = wsum > 0.0 ? (float)(1.0 / wsum) : 0.f;
+    SPH_HIP(c, launch_mlp_forward(c->stream, w, P));
+    SPH_HIP(c, launch_mlp_aggregate(c->stream, w, T, inv_wsum));
+    SPH_HIP(c, launch_mlp_backward(c->stream, w, P, R, (uint32_t)nblocks));
+    std::vector<double> lossp(T);
+    SPH_HIP(c, hipMemcpyAsync(lossp.data(), w.lossp, sizeof(double) * T, hipMemcpyDeviceToHost, c->stream));
+    if (grad_out)
+        SPH_HIP(c, hipMemcpyAsync(grad_out, w.grad, sizeof(float) * kMlpParams, hipMemcpyDeviceToHost, c->stream));
+    if (o->update) {
+        c->adam_step += 1;
+        const float bc1 = 1.f - std::pow(o->beta1, (float)c->adam_step);
+        const float bc2 = 1.f - std::pow(o->beta2, (float)c->adam_step);
+        SPH_HIP(c, launch_mlp_adam(c->stream, w, o->lr, o->beta1, o->beta2, o->eps, bc1, bc2));
+    }
+    SPH_HIP(c, hipStreamSynchronize(c->stream));
+    double L = 0.0;
+    for (double x : lossp) L += x;
+    if (res) {
+        res->loss = wsum > 0.0 ? L / wsum : 0.0;
+        res->pairs = P;
+        res->targets = T;
+        res->weighted = (uint32_t)wsum;
+    }
+    return SPH_OK;
+}
+
+int sph_mlp_predict(sph_ctx* c, int path, float* out)
+{
+    if (!c) return SPH_ERR_INVALID;
+    if (!out) return fail(c, SPH_ERR_INVALID, "out is null");
+    if (path != 0 && path != 1) return fail(c, SPH_ERR_INVALID, "path must be 0 (FP32) or 1 (MFMA)");
+    if (!c->mlp_loaded) return fail(c, SPH_ERR_STATE, "no network loaded (sph_mlp_set)");
+    int rc = set_device(c);
+    if (rc != SPH_OK) return rc;
+    if ((rc = density_pass(c, true)) != SPH_OK) return rc;
+    if ((rc = ensure_targets(c, c->n)) != SPH_OK) return rc;
+    if (!c->d_pred_aos && dalloc(&c->d_pred_aos, (size_t)c->n * 3u) != hipSuccess) {
+        (void)hipGetLastError();
+        c->d_pred_aos = nullptr;
+        return fail(c, SPH_ERR_NOMEM, "cannot allocate prediction buffer");
+    }
+    SPH_RUN(c, "mlp_messages", launch_coarse_mlp(c->stream, c->d, c->mlp, path));
+    SPH_HIP(c, launch_pred_to_aos(c->stream, c->d, c->mlp, c->d_pred_aos));
+    SPH_HIP(c, hipMemcpyAsync(out, c->d_pred_aos, sizeof(float) * 3u * c->n, hipMemcpyDeviceToHost, c->stream));
+    SPH_HIP(c, hipStreamSynchronize(c->stream));
+    return SPH_OK;
+}
+
+namespace {
+
+int mean_rho(sph_ctx* c, double* out)
+{
+    double tot = 0.0;
+    for (int s = 0; s < c->nscenes; ++s) {
+        sph_metrics_t m;
+        int rc = sph_metrics(c, s, &m);
+        if (rc != SPH_OK) return rc;
+        tot += m.mean_rho * (double)c->count[s];
+    }
+    *out = tot / (double)c->n;
+    return SPH_OK;
+}
+
+// One coarse step from the current AoS state.
+int coarse_once(sph_ctx* c, bool use_network)
+{
+    int rc;
+    if ((rc = density_pass(c, false)) != SPH_OK) return rc;
+    if (use_network) {
+        SPH_RUN(c, "mlp_messages", launch_coarse_mlp(c->stream, c->d, c->mlp, 1));
+    } else {
+        SPH_RUN(c, "force", launch_force(c->stream, c->d, false));
+    }
+    SPH_RUN(c, "coarse_integrate", launch_coarse_integrate(c->stream, c->d, c->mlp, !use_network));
+    c->b_valid = true;
+    c->a_valid = false;
+    c->rho_valid = false;
+    return SPH_OK;
+}
+
+int copy_aos(sph_ctx* c, sph_particle_t* dst, const sph_particle_t* src)
+{
+    SPH_HIP(c, hipMemcpyAsync(dst, src, sizeof(sph_particle_t) * c->n, hipMemcpyDeviceToDevice, c->stream));
+    return SPH_OK;
+}
+
+}  // namespace
+
+int sph_coarse_step(sph_ctx* c, int n, const sph_coarse_opts_t* o, sph_coarse_stats_t* st)
+{
+    if (!c) return SPH_ERR_INVALID;
+    if (!o || n < 0 || o->judge_every < 0 || o->cooldown < 0 || !(o->drift_limit >= 0.f))
+        return fail(c, SPH_ERR_INVALID, "bad coarse-step options");
+    if (o->use_network && !c->mlp_loaded) return fail(c, SPH_ERR_STATE, "no network loaded (sph_mlp_set)");
+    int rc = set_device(c);
+    if (rc != SPH_OK) return rc;
+    if ((rc = ensure_targets(c, c->n)) != SPH_OK) return rc;  // per-slot predictions live in mlp.pred
+    if (o->judge_every > 0 && (!c->d_judge_s || !c->d_judge_p)) {
+        if (dalloc(&c->d_judge_s, c->n) != hipSuccess || dalloc(&c->d_judge_p, c->n) != hipSuccess) {
+            (void)hipGetLastError();
+            return fail(c, SPH_ERR_NOMEM, "cannot allocate judge buffers");
+        }
+    }
+    sph_coarse_stats_t local{};
+    sph_coarse_stats_t& s = st ? *st : local;
+    s = sph_coarse_stats_t{};
+    int cooldown = 0;
+    for (int k = 0; k < n; ++k) {
+        if (cooldown > 0) {
+            --cooldown;
+            if ((rc = sph_step(c, 4)) != SPH_OK) return rc;
+            s.plain_steps += 4;
+            continue;
+        }
+        const bool judge = o->judge_every > 0 && (k % o->judge_every) == 0;
+        if (!judge) {
+            if ((rc = coarse_once(c, o->use_network != 0)) != SPH_OK) return rc;
+            s.coarse_steps += 1;
+            continue;
+        }
+        // Judge: 4 plain steps and one coarse step from the same state.
+        double mp = 0.0, mc = 0.0;
+        if ((rc = copy_aos(c, c->d_judge_s, c->d.aos)) != SPH_OK) return rc;
+        invalidate_state(c);
+        if ((rc = sph_step(c, 4)) != SPH_OK) return rc;
+        if ((rc = density_pass(c, true)) != SPH_OK) return rc;
+        if ((rc = mean_rho(c, &mp)) != SPH_OK) return rc;
+        if ((rc = copy_aos(c, c->d_judge_p, c->d.aos)) != SPH_OK) return rc;
+        if ((rc = copy_aos(c, c->d.aos, c->d_judge_s)) != SPH_OK) return rc;
+        invalidate_state(c);
+        if ((rc = coarse_once(c, o->use_network != 0)) != SPH_OK) return rc;
+        if ((rc = density_pass(c, true)) != SPH_OK) return rc;
+        if ((rc = mean_rho(c, &mc)) != SPH_OK) return rc;
+        const double drift = std::fabs(mc - mp);
+        s.judged += 1;
+        s.last_drift = drift;
+        s.max_drift = std::max(s.max_drift, drift);
+        if (!(drift <= (double)o->drift_limit)) {
+            if ((rc = copy_aos(c, c->d.aos, c->d_judge_p)) != SPH_OK) return rc;
+            invalidate_state(c);
+            s.fallbacks += 1;
+            s.plain_steps += 4;
+            cooldown = o->cooldown;
+        } else {
+            s.coarse_steps += 1;
+        }
+    }
     return SPH_OK;
 }
 

@@ -50,6 +50,52 @@ struct DeviceState {
 
 bool tuning_supported(const sph_tuning_t& t);
 
+// ---- learned coarse step -------------------------------------------------
+
+constexpr int kMlpIn = 9, kMlpHidden = 64, kMlpOut = 3;
+constexpr int kMlpParams = kMlpIn * kMlpHidden + kMlpHidden + kMlpHidden * kMlpHidden + kMlpHidden +
+                           kMlpHidden * kMlpOut + kMlpOut;  // 4995
+
+// GELU as 0.5x + q(min(x^2, 9)) + 0.5*max(|x|-3, 0), q(s) = s(c0 + s(c1 + s(c2 + s c3))):
+// a transcendental-free form of x*Phi(x), max |error| 2.8e-3.
+constexpr float kGeluC0 = 3.88549402e-01f, kGeluC1 = -5.37665106e-02f, kGeluC2 = 4.76835308e-03f,
+                kGeluC3 = -1.70698904e-04f, kGeluClamp = 9.0f, kGeluKnee = 3.0f;
+
+struct MlpNorm {
+    float v_scale;    // velocity difference scale
+    float rho_scale;  // density scale
+    float pi_scale;   // pressure-term scale
+    float acc_scale;  // acceleration per unit of summed message
+};
+
+// Device buffers of the MLP (parameters, optimiser state) and of the current
+// materialised pair batch.  Grown on demand; pair buffers hold `pair_cap`
+// rows, target buffers `target_cap` entries.
+struct MlpWork {
+    MlpNorm norm;
+    float* params = nullptr;  // kMlpParams float32 master weights
+    float* adam_m = nullptr;
+    float* adam_v = nullptr;
+    float* grad = nullptr;
+    uint64_t pair_cap = 0, target_cap = 0, partial_cap = 0;
+    uint32_t* cnt = nullptr;   // per target
+    uint64_t* off = nullptr;   // per target + 1
+    float* tgt = nullptr;      // 3 per target
+    float* wt = nullptr;       // per target
+    float* pred = nullptr;     // 3 per target (or per slot for the coarse step)
+    float* dpred = nullptr;    // 3 per target
+    double* lossp = nullptr;   // per target
+    uint32_t* pj = nullptr;    // per pair: neighbour slot
+    uint32_t* pt = nullptr;    // per pair: target index
+    float* X = nullptr;        // 9 per pair
+    float* Z1 = nullptr;       // 64 per pair
+    float* Z2 = nullptr;
+    float* dZ1 = nullptr;
+    float* dZ2 = nullptr;
+    float* M = nullptr;        // 3 per pair
+    float* partial = nullptr;  // kMlpParams per gradient block
+};
+
 // Each launcher enqueues one kernel on `s` and returns hipGetLastError().
 hipError_t launch_set_scene(hipStream_t s, SceneDev* dst, const SceneDev& value);
 hipError_t launch_fill_aos_defaults(hipStream_t s, sph_particle_t* aos, uint32_t n);
@@ -67,5 +113,18 @@ hipError_t launch_density(hipStream_t s, const DeviceState& d, bool write_aos);
 hipError_t launch_force(hipStream_t s, const DeviceState& d, bool fuse_integrate);
 hipError_t launch_integrate_aos(hipStream_t s, const DeviceState& d);
 hipError_t launch_metrics(hipStream_t s, const DeviceState& d, int first_tile, int ntiles);
+
+hipError_t launch_pairs(hipStream_t s, const DeviceState& d, int nscenes, const MlpWork& w, uint32_t first,
+                        uint32_t stride, uint32_t T, bool emit);
+hipError_t launch_mlp_forward(hipStream_t s, const MlpWork& w, uint64_t P);
+hipError_t launch_mlp_aggregate(hipStream_t s, const MlpWork& w, uint32_t T, float inv_wsum);
+hipError_t launch_mlp_backward(hipStream_t s, const MlpWork& w, uint64_t P, uint32_t rows_per_block, uint32_t nblocks);
+hipError_t launch_mlp_adam(hipStream_t s, const MlpWork& w, float lr, float beta1, float beta2, float eps, float bc1,
+                           float bc2);
+hipError_t launch_train_targets(hipStream_t s, const DeviceState& d, int nscenes, const MlpWork& w,
+                                const sph_particle_t* later, uint32_t first, uint32_t stride, uint32_t T);
+hipError_t launch_coarse_integrate(hipStream_t s, const DeviceState& d, const MlpWork& w, bool use_force);
+hipError_t launch_coarse_mlp(hipStream_t s, const DeviceState& d, const MlpWork& w, int path);
+hipError_t launch_pred_to_aos(hipStream_t s, const DeviceState& d, const MlpWork& w, float* out);
 
 }  // namespace nprsph

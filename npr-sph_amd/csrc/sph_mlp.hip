@@ -1,0 +1,597 @@
+// sph_mlp.hip - kernels of the learned coarse step (no reference counterpart).
+//
+// Network: a pair-message MLP 9 -> 64 -> GELU -> 64 -> GELU -> 3 over every
+// neighbour pair within h.  Messages are summed per particle and scaled to an
+// acceleration; one coarse step advances 4 fine steps.  Parameter layout
+// (float32, 4995 values): W1[9][64], b1[64], W2[64][64], b2[64], W3[64][3], b3[3].
+//
+// Training runs on materialised pair batches in FP32:
+//   k_pairs_count / k_pairs_emit  neighbour pairs + features of a target set
+//   k_mlp_forward_rows            z1, z2 and messages per pair
+//   k_mlp_aggregate               per-target sums, loss terms and dL/dpred
+//   k_mlp_backward_rows           dz2, dz1 per pair
+//   k_mlp_wgrad_partial/_reduce   weight gradients as block partial sums
+//                                 added in a fixed order (deterministic)
+//   k_mlp_adam                    Adam update of the float32 master weights
+// The coarse step integrates with k_coarse_integrate.
+#include "sph_gather.h"
+#include "sph_internal.h"
+
+namespace nprsph {
+
+namespace {
+
+constexpr int kNin = kMlpIn, kNh = kMlpHidden, kNout = kMlpOut;
+constexpr int kOffW1 = 0;
+constexpr int kOffB1 = kOffW1 + kNin * kNh;
+constexpr int kOffW2 = kOffB1 + kNh;
+constexpr int kOffB2 = kOffW2 + kNh * kNh;
+constexpr int kOffW3 = kOffB2 + kNh;
+constexpr int kOffB3 = kOffW3 + kNh * kNout;
+static_assert(kOffB3 + kNout == kMlpParams, "parameter layout");
+
+// Row layout of one pair in the gradient staging buffer:
+// [x(9), 1, h1(64), h2(64), dz1(64), dz2(64), dm(3)].
+constexpr int kRowX = 0, kRowOne = 9, kRowH1 = 10, kRowH2 = kRowH1 + kNh, kRowDz1 = kRowH2 + kNh,
+              kRowDz2 = kRowDz1 + kNh, kRowDm = kRowDz2 + kNh, kRowLen = kRowDm + kNout;
+constexpr int kGradRows = 16;  // rows staged in LDS per sub-chunk
+
+__device__ __forceinline__ uint32_t scene_of(const SceneDev* __restrict__ scenes, int nscenes, uint32_t s)
+{
+    int lo = 0, hi = nscenes - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (s < scenes[mid].p_end)
+            hi = mid;
+        else
+            lo = mid + 1;
+    }
+    return (uint32_t)lo;
+}
+
+// Features of pair (i = slot s, j) in float32 (the pressure term is the
+// force_comp.glsl:59 factor (p_i+p_j)/(2 rho_j)).
+__device__ __forceinline__ void pair_features(const SceneDev& sc, const MlpNorm& nm, float4 xi, float4 vi, float rhoi,
+                                              float pi, float4 xj, float4 vj, float rhoj, float2 dj, float f[kNin])
+{
+    const float inv_h = 1.0f / sc.h;
+    f[0] = (xi.x - xj.x) * inv_h;
+    f[1] = (xi.y - xj.y) * inv_h;
+    f[2] = (xi.z - xj.z) * inv_h;
+    f[3] = (vi.x - vj.x) * nm.v_scale;
+    f[4] = (vi.y - vj.y) * nm.v_scale;
+    f[5] = (vi.z - vj.z) * nm.v_scale;
+    f[6] = rhoi * nm.rho_scale;
+    f[7] = rhoj * nm.rho_scale;
+    f[8] = (pi + dj.x) * (0.5f * dj.y) * nm.pi_scale;
+}
+
+template <int SUB, bool EMIT>
+__global__ __launch_bounds__(256) void k_pairs(const SceneDev* __restrict__ scenes, int nscenes,
+                                               const uint32_t* __restrict__ start, const float4* __restrict__ P,
+                                               const float4* __restrict__ V, const float* __restrict__ rho,
+                                               const float2* __restrict__ D2, uint32_t first, uint32_t stride,
+                                               uint32_t T, MlpNorm nm, uint32_t* __restrict__ cnt,
+                                               const uint64_t* __restrict__ off, uint32_t* __restrict__ pj,
+                                               uint32_t* __restrict__ pt, float* __restrict__ X)
+{
+    __shared__ uint32_t rows[2 * (2 * SUB + 1) * 256];
+    const uint32_t t = blockIdx.x * 256u + threadIdx.x;
+    if (t >= T) return;
+    const uint32_t s = first + t * stride;
+    const SceneDev& sc = scenes[scene_of(scenes, nscenes, s)];
+    const float4 xi = P[s];
+    const float h2 = sc.h2;
+    uint32_t count = 0;
+    uint64_t row = EMIT ? off[t] : 0;
+    float4 vi;
+    float rhoi = 0.f, pi = 0.f;
+    if constexpr (EMIT) {
+        vi = V[s];
+        rhoi = rho[s];
+        pi = D2[s].x;
+    }
+    int cx, cy, cz;
+    float fx, fy, fz;
+    if (cell_coords(sc, xi.x, xi.y, xi.z, fx, fy, fz, cx, cy, cz)) {
+        for_each_row<SUB, true, 256>(sc, start, fx, fy, fz, cx, cy, cz, rows + threadIdx.x, [&](uint32_t b, uint32_t e) {
+            for (uint32_t j = b; j < e; ++j) {
+                const float4 xj = P[j];
+                const float ddx = xi.x - xj.x, ddy = xi.y - xj.y, ddz = xi.z - xj.z;
+                const float r2 = fmaf(ddx, ddx, fmaf(ddy, ddy, ddz * ddz));
+                if (r2 < h2 && j != s) {
+                    if constexpr (EMIT) {
+                        float f[kNin];
+                        pair_features(sc, nm, xi, vi, rhoi, pi, xj, V[j], rho[j], D2[j], f);
+#pragma unroll
+                        for (int k = 0; k < kNin; ++k) X[row * kNin + k] = f[k];
+                        pj[row] = j;
+                        pt[row] = t;
+                        ++row;
+                    } else {
+                        ++count;
+                    }
+                }
+            }
+        });
+    }
+    if constexpr (!EMIT) cnt[t] = count;
+}
+
+__device__ __forceinline__ float gelu_poly(float x)
+{
+    const float s = fminf(x * x, kGeluClamp);
+    const float q = fmaf(fmaf(fmaf(kGeluC3, s, kGeluC2), s, kGeluC1), s, kGeluC0) * s;
+    return fmaf(0.5f, x, q) + 0.5f * fmaxf(fabsf(x) - kGeluKnee, 0.f);
+}
+
+__device__ __forceinline__ float gelu_poly_grad(float x)
+{
+    const float s = x * x;
+    const float dq = fmaf(fmaf(fmaf(4.f * kGeluC3, s, 3.f * kGeluC2), s, 2.f * kGeluC1), s, kGeluC0);
+    float g = 0.5f;
+    if (s < kGeluClamp) g = fmaf(dq * 2.f, x, g);
+    if (fabsf(x) > kGeluKnee) g += copysignf(0.5f, x);
+    return g;
+}
+
+__device__ __forceinline__ void load_params_lds(float* lds, const float* __restrict__ params)
+{
+    for (int i = threadIdx.x; i < kMlpParams; i += blockDim.x) lds[i] = params[i];
+    __syncthreads();
+}
+
+__global__ __launch_bounds__(256) void k_mlp_forward_rows(const float* __restrict__ params, const float* __restrict__ X,
+                                                          uint64_t P, float* __restrict__ Z1, float* __restrict__ Z2,
+                                                          float* __restrict__ M)
+{
+    __shared__ float w[kMlpParams];
+    load_params_lds(w, params);
+    const uint64_t r = (uint64_t)blockIdx.x * 256u + threadIdx.x;
+    if (r >= P) return;
+    float f[kNin];
+#pragma unroll
+    for (int k = 0; k < kNin; ++k) f[k] = X[r * kNin + k];
+    float h1[kNh];
+#pragma unroll
+    for (int u = 0; u < kNh; ++u) {
+        float z = w[kOffB1 + u];
+#pragma unroll
+        for (int k = 0; k < kNin; ++k) z = fmaf(f[k], w[kOffW1 + k * kNh + u], z);
+        Z1[r * kNh + u] = z;
+        h1[u] = gelu_poly(z);
+    }
+    float m[kNout] = {w[kOffB3], w[kOffB3 + 1], w[kOffB3 + 2]};
+    for (int u = 0; u < kNh; ++u) {
+        float z = w[kOffB2 + u];
+#pragma unroll
+        for (int k = 0; k < kNh; ++k) z = fmaf(h1[k], w[kOffW2 + k * kNh + u], z);
+        Z2[r * kNh + u] = z;
+        const float h = gelu_poly(z);
+#pragma unroll
+        for (int c = 0; c < kNout; ++c) m[c] = fmaf(h, w[kOffW3 + u * kNout + c], m[c]);
+    }
+#pragma unroll
+    for (int c = 0; c < kNout; ++c) M[r * kNout + c] = m[c];
+}
+
+// pred_t = sum of the target's messages (pair order); loss term and dL/dpred.
+__global__ __launch_bounds__(256) void k_mlp_aggregate(const float* __restrict__ M, const uint64_t* __restrict__ off,
+                                                       uint32_t T, const float* __restrict__ tgt,
+                                                       const float* __restrict__ wt, float inv_wsum,
+                                                       float* __restrict__ pred, float* __restrict__ dpred,
+                                                       double* __restrict__ lossp)
+{
+    const uint32_t t = blockIdx.x * 256u + threadIdx.x;
+    if (t >= T) return;
+    float a[kNout] = {0.f, 0.f, 0.f};
+    for (uint64_t r = off[t]; r < off[t + 1]; ++r)
+#pragma unroll
+        for (int c = 0; c < kNout; ++c) a[c] += M[r * kNout + c];
+    double l = 0.0;
+#pragma unroll
+    for (int c = 0; c < kNout; ++c) {
+        pred[t * kNout + c] = a[c];
+        const float d = a[c] - tgt[t * kNout + c];
+        dpred[t * kNout + c] = 2.f * wt[t] * d * inv_wsum;
+        l += (double)wt[t] * (double)d * (double)d;
+    }
+    lossp[t] = l;
+}
+
+__global__ __launch_bounds__(256) void k_mlp_backward_rows(const float* __restrict__ params, const float* __restrict__ Z1,
+                                                           const float* __restrict__ Z2, const uint32_t* __restrict__ pt,
+                                                           const float* __restrict__ dpred, uint64_t P,
+                                                           float* __restrict__ dZ1, float* __restrict__ dZ2)
+{
+    __shared__ float w[kMlpParams];
+    load_params_lds(w, params);
+    const uint64_t r = (uint64_t)blockIdx.x * 256u + threadIdx.x;
+    if (r >= P) return;
+    const uint32_t t = pt[r];
+    const float dm[kNout] = {dpred[t * kNout], dpred[t * kNout + 1], dpred[t * kNout + 2]};
+    float dz2[kNh];
+#pragma unroll
+    for (int k = 0; k < kNh; ++k) {
+        float d = 0.f;
+#pragma unroll
+        for (int c = 0; c < kNout; ++c) d = fmaf(w[kOffW3 + k * kNout + c], dm[c], d);
+        dz2[k] = d * gelu_poly_grad(Z2[r * kNh + k]);
+        dZ2[r * kNh + k] = dz2[k];
+    }
+    for (int k = 0; k < kNh; ++k) {
+        float d = 0.f;
+#pragma unroll
+        for (int u = 0; u < kNh; ++u) d = fmaf(w[kOffW2 + k * kNh + u], dz2[u], d);
+        dZ1[r * kNh + k] = d * gelu_poly_grad(Z1[r * kNh + k]);
+    }
+}
+
+// Block b sums, over its rows [b*R, (b+1)*R), the outer products that make up
+// every parameter's gradient; partial[b][p].
+__global__ __launch_bounds__(256) void k_mlp_wgrad_partial(const float* __restrict__ X, const float* __restrict__ Z1,
+                                                           const float* __restrict__ Z2, const float* __restrict__ dZ1,
+                                                           const float* __restrict__ dZ2, const uint32_t* __restrict__ pt,
+                                                           const float* __restrict__ dpred, uint64_t P, uint32_t R,
+                                                           float* __restrict__ partial)
+{
+    constexpr int kOwn = (kMlpParams + 255) / 256;
+    __shared__ float rowbuf[kGradRows * kRowLen];
+    int ia[kOwn], ib[kOwn];
+    float acc[kOwn];
+#pragma unroll
+    for (int q = 0; q < kOwn; ++q) {
+        const int p = threadIdx.x + 256 * q;
+        acc[q] = 0.f;
+        ia[q] = kRowOne;
+        ib[q] = kRowOne;
+        if (p < kOffB1) {
+            ia[q] = kRowX + p / kNh;
+            ib[q] = kRowDz1 + p % kNh;
+        } else if (p < kOffW2) {
+            ib[q] = kRowDz1 + (p - kOffB1);
+        } else if (p < kOffB2) {
+            ia[q] = kRowH1 + (p - kOffW2) / kNh;
+            ib[q] = kRowDz2 + (p - kOffW2) % kNh;
+        } else if (p < kOffW3) {
+            ib[q] = kRowDz2 + (p - kOffB2);
+        } else if (p < kOffB3) {
+            ia[q] = kRowH2 + (p - kOffW3) / kNout;
+            ib[q] = kRowDm + (p - kOffW3) % kNout;
+        } else if (p < kMlpParams) {
+            ib[q] = kRowDm + (p - kOffB3);
+        }
+    }
+    const uint64_t r0 = (uint64_t)blockIdx.x * R;
+    const uint64_t r1 = r0 + R < P ? r0 + R : P;
+    for (uint64_t rb = r0; rb < r1; rb += kGradRows) {
+        const int nr = (int)((r1 - rb) < (uint64_t)kGradRows ? (r1 - rb) : (uint64_t)kGradRows);
+        __syncthreads();
+        for (int i = threadIdx.x; i < kGradRows * kRowLen; i += 256) {
+            const int rr = i / kRowLen, c = i % kRowLen;
+            float v = 0.f;
+            if (rr < nr) {
+                const uint64_t r = rb + rr;
+                if (c < kRowOne)
+                    v = X[r * kNin + c];
+                else if (c == kRowOne)
+                    v = 1.f;
+                else if (c < kRowH2)
+                    v = gelu_poly(Z1[r * kNh + (c - kRowH1)]);
+                else if (c < kRowDz1)
+                    v = gelu_poly(Z2[r * kNh + (c - kRowH2)]);
+                else if (c < kRowDz2)
+                    v = dZ1[r * kNh + (c - kRowDz1)];
+                else if (c < kRowDm)
+                    v = dZ2[r * kNh + (c - kRowDz2)];
+                else
+                    v = dpred[pt[r] * kNout + (c - kRowDm)];
+            }
+            rowbuf[i] = v;
+        }
+        __syncthreads();
+        for (int rr = 0; rr < nr; ++rr) {
+            const float* row = rowbuf + rr * kRowLen;
+#pragma unroll
+            for (int q = 0; q < kOwn; ++q) acc[q] = fmaf(row[ia[q]], row[ib[q]], acc[q]);
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < kOwn; ++q) {
+        const int p = threadIdx.x + 256 * q;
+        if (p < kMlpParams) partial[(size_t)blockIdx.x * kMlpParams + p] = acc[q];
+    }
+}
+
+__global__ __launch_bounds__(256) void k_mlp_wgrad_reduce(const float* __restrict__ partial, uint32_t nblocks,
+                                                          float* __restrict__ grad)
+{
+    const int p = blockIdx.x * 256 + threadIdx.x;
+    if (p >= kMlpParams) return;
+    double s = 0.0;
+    for (uint32_t b = 0; b < nblocks; ++b) s += (double)partial[(size_t)b * kMlpParams + p];
+    grad[p] = (float)s;
+}
+
+// Adam (Kingma and Ba, 2015) with bias correction, float32 master weights.
+__global__ __launch_bounds__(256) void k_mlp_adam(float* __restrict__ params, const float* __restrict__ grad,
+                                                  float* __restrict__ m, float* __restrict__ v, float lr, float beta1,
+                                                  float beta2, float eps, float bc1, float bc2)
+{
+    const int p = blockIdx.x * 256 + threadIdx.x;
+    if (p >= kMlpParams) return;
+    const float g = grad[p];
+    const float mn = beta1 * m[p] + (1.f - beta1) * g;
+    const float vn = beta2 * v[p] + (1.f - beta2) * g * g;
+    m[p] = mn;
+    v[p] = vn;
+    params[p] -= lr * (mn / bc1) / (sqrtf(vn / bc2) + eps);
+}
+
+// Training targets of target t (slot first + t*stride): the mean acceleration
+// over the next 4 fine steps, (v(t+4dt) - v(t))/(4 dt) - G, in units of
+// acc_scale; weight 0 for particles that are non-finite or within 1e-6 of a
+// wall in either state (the wall clamp is not part of the learned update).
+__global__ __launch_bounds__(256) void k_train_targets(const SceneDev* __restrict__ scenes, int nscenes,
+                                                       const uint32_t* __restrict__ perm,
+                                                       const sph_particle_t* __restrict__ now,
+                                                       const sph_particle_t* __restrict__ later, uint32_t first,
+                                                       uint32_t stride, uint32_t T, float inv_acc_scale,
+                                                       float* __restrict__ tgt, float* __restrict__ wt)
+{
+    const uint32_t t = blockIdx.x * 256u + threadIdx.x;
+    if (t >= T) return;
+    const uint32_t s = first + t * stride;
+    const SceneDev& sc = scenes[scene_of(scenes, nscenes, s)];
+    const uint32_t o = perm[s];
+    const sph_particle_t& a = now[o];
+    const sph_particle_t& b = later[o];
+    bool ok = true;
+    float out[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const float acc = (b.vel[c] - a.vel[c]) / (4.f * sc.dt) - sc.grav[c];
+        out[c] = acc * inv_acc_scale;
+        ok = ok && isfinite(out[c]) && isfinite(a.pos[c]) && isfinite(b.pos[c]);
+        const float p0 = a.pos[c], p1 = b.pos[c];
+        ok = ok && fabsf(p0 - sc.lower[c]) > 1e-6f && fabsf(p0 - sc.upper[c]) > 1e-6f &&
+             fabsf(p1 - sc.lower[c]) > 1e-6f && fabsf(p1 - sc.upper[c]) > 1e-6f;
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) tgt[t * 3 + c] = ok ? out[c] : 0.f;
+    wt[t] = ok ? 1.f : 0.f;
+}
+
+// Coarse step for every slot: a = acc_scale*pred + G (network) or a = F/rho
+// (SPH force held over the 4 fine steps); v' = v + 4dt a,
+// x' = x + 4dt v + 10 dt^2 a, then the wall clamp of integrate_comp.glsl:46-77.
+// Writes the AoS record (pos, vel, force = rho*a, extras.xy) and state B.
+__global__ __launch_bounds__(kTile) void k_coarse_integrate(const Tile* __restrict__ tiles,
+                                                            const SceneDev* __restrict__ scenes,
+                                                            const float4* __restrict__ PA, const float4* __restrict__ VA,
+                                                            const float* __restrict__ rho, const float2* __restrict__ D2,
+                                                            const float* __restrict__ pred,
+                                                            const uint32_t* __restrict__ perm,
+                                                            sph_particle_t* __restrict__ aos, float4* __restrict__ PB,
+                                                            float4* __restrict__ VB, float acc_scale, int use_force)
+{
+#pragma clang fp contract(off)
+    const Tile t = tiles[blockIdx.x];
+    const SceneDev& sc = scenes[t.scene];
+    const uint32_t s = t.begin + threadIdx.x;
+    if (s >= t.end) return;
+    const float4 x = PA[s], v = VA[s];
+    const float rh = rho[s];
+    float* r = reinterpret_cast<float*>(aos + perm[s]);
+    float a[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) a[c] = use_force ? r[8 + c] / rh : acc_scale * pred[s * 3 + c] + sc.grav[c];
+    const float xs[3] = {x.x, x.y, x.z}, vs[3] = {v.x, v.y, v.z};
+    const float dt4 = 4.f * sc.dt, dt2_10 = 10.f * sc.dt * sc.dt;
+    float nx[3], nv[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        float vel = vs[c] + dt4 * a[c];
+        float pos = xs[c] + dt4 * vs[c] + dt2_10 * a[c];
+        if (pos < sc.lower[c]) {
+            pos = sc.lower[c];
+            vel *= -sc.damping;
+        } else if (pos > sc.upper[c]) {
+            pos = sc.upper[c];
+            vel *= -sc.damping;
+        }
+        nx[c] = pos;
+        nv[c] = vel;
+    }
+    PB[s] = make_float4(nx[0], nx[1], nx[2], 0.f);
+    VB[s] = make_float4(nv[0], nv[1], nv[2], 0.f);
+    r[0] = nx[0];
+    r[1] = nx[1];
+    r[2] = nx[2];
+    r[4] = nv[0];
+    r[5] = nv[1];
+    r[6] = nv[2];
+    r[8] = rh * a[0];
+    r[9] = rh * a[1];
+    r[10] = rh * a[2];
+    r[12] = rh;
+    r[13] = D2[s].x;
+}
+
+// Summed messages of every slot, FP32, one particle per lane and the whole MLP
+// per pair in registers (weights in LDS).  The float32 reference of the
+// coarse step; the fast path is the MFMA kernel.
+template <int SUB>
+__global__ __launch_bounds__(256) void k_coarse_mlp_fp32(const Tile* __restrict__ tiles,
+                                                         const SceneDev* __restrict__ scenes,
+                                                         const uint32_t* __restrict__ start,
+                                                         const float4* __restrict__ P, const float4* __restrict__ V,
+                                                         const float* __restrict__ rho, const float2* __restrict__ D2,
+                                                         const float* __restrict__ params, MlpNorm nm,
+                                                         float* __restrict__ pred)
+{
+    __shared__ float w[kMlpParams];
+    __shared__ uint32_t rows[2 * (2 * SUB + 1) * 256];
+    load_params_lds(w, params);
+    const Tile t = tiles[blockIdx.x];
+    const SceneDev& sc = scenes[t.scene];
+    const uint32_t s = t.begin + threadIdx.x;
+    if (s >= t.end) return;
+    const float4 xi = P[s], vi = V[s];
+    const float rhoi = rho[s], pi = D2[s].x;
+    const float h2 = sc.h2;
+    float acc[kNout] = {0.f, 0.f, 0.f};
+    int cx, cy, cz;
+    float fx, fy, fz;
+    if (cell_coords(sc, xi.x, xi.y, xi.z, fx, fy, fz, cx, cy, cz)) {
+        for_each_row<SUB, true, 256>(sc, start, fx, fy, fz, cx, cy, cz, rows + threadIdx.x, [&](uint32_t b, uint32_t e) {
+            for (uint32_t j = b; j < e; ++j) {
+                const float4 xj = P[j];
+                const float ddx = xi.x - xj.x, ddy = xi.y - xj.y, ddz = xi.z - xj.z;
+                const float r2 = fmaf(ddx, ddx, fmaf(ddy, ddy, ddz * ddz));
+                if (!(r2 < h2) || j == s) continue;
+                float f[kNin];
+                pair_features(sc, nm, xi, vi, rhoi, pi, xj, V[j], rho[j], D2[j], f);
+                float h1[kNh];
+#pragma unroll
+                for (int u = 0; u < kNh; ++u) {
+                    float z = w[kOffB1 + u];
+#pragma unroll
+                    for (int k = 0; k < kNin; ++k) z = fmaf(f[k], w[kOffW1 + k * kNh + u], z);
+                    h1[u] = gelu_poly(z);
+                }
+                float m[kNout] = {w[kOffB3], w[kOffB3 + 1], w[kOffB3 + 2]};
+                for (int u = 0; u < kNh; ++u) {
+                    float z = w[kOffB2 + u];
+#pragma unroll
+                    for (int k = 0; k < kNh; ++k) z = fmaf(h1[k], w[kOffW2 + k * kNh + u], z);
+                    const float hh = gelu_poly(z);
+#pragma unroll
+                    for (int c = 0; c < kNout; ++c) m[c] = fmaf(hh, w[kOffW3 + u * kNout + c], m[c]);
+                }
+#pragma unroll
+                for (int c = 0; c < kNout; ++c) acc[c] += m[c];
+            }
+        });
+    }
+#pragma unroll
+    for (int c = 0; c < kNout; ++c) pred[s * 3 + c] = acc[c];
+}
+
+// pred of slot s scattered to AoS order (diagnostics).
+__global__ __launch_bounds__(256) void k_pred_to_aos(const float* __restrict__ pred, const uint32_t* __restrict__ perm,
+                                                     uint32_t n, float* __restrict__ out)
+{
+    const uint32_t s = blockIdx.x * 256u + threadIdx.x;
+    if (s >= n) return;
+    const uint32_t o = perm[s];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) out[o * 3 + c] = pred[s * 3 + c];
+}
+
+inline unsigned blocks256(uint64_t n) { return (unsigned)((n + 255u) / 256u); }
+
+}  // namespace
+
+hipError_t launch_pairs(hipStream_t st, const DeviceState& d, int nscenes, const MlpWork& w, uint32_t first,
+                        uint32_t stride, uint32_t T, bool emit)
+{
+    if (T == 0) return hipSuccess;
+    const dim3 grid(blocks256(T)), block(256);
+    const int sub = d.tuning.cells_per_h;
+#define NPRSPH_PAIRS(S)                                                                                          \
+    if (emit)                                                                                                    \
+        hipLaunchKernelGGL((k_pairs<S, true>), grid, block, 0, st, d.scenes, nscenes, d.start, d.PA, d.VA, d.rho, \
+                           d.D2, first, stride, T, w.norm, w.cnt, w.off, w.pj, w.pt, w.X);                       \
+    else                                                                                                         \
+        hipLaunchKernelGGL((k_pairs<S, false>), grid, block, 0, st, d.scenes, nscenes, d.start, d.PA, d.VA, d.rho, \
+                           d.D2, first, stride, T, w.norm, w.cnt, w.off, w.pj, w.pt, w.X);
+    if (sub == 1) {
+        NPRSPH_PAIRS(1)
+    } else if (sub == 2) {
+        NPRSPH_PAIRS(2)
+    } else {
+        NPRSPH_PAIRS(3)
+    }
+#undef NPRSPH_PAIRS
+    return hipGetLastError();
+}
+
+hipError_t launch_mlp_forward(hipStream_t st, const MlpWork& w, uint64_t P)
+{
+    if (P == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_mlp_forward_rows, dim3(blocks256(P)), dim3(256), 0, st, w.params, w.X, P, w.Z1, w.Z2, w.M);
+    return hipGetLastError();
+}
+
+hipError_t launch_mlp_aggregate(hipStream_t st, const MlpWork& w, uint32_t T, float inv_wsum)
+{
+    if (T == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_mlp_aggregate, dim3(blocks256(T)), dim3(256), 0, st, w.M, w.off, T, w.tgt, w.wt, inv_wsum,
+                       w.pred, w.dpred, w.lossp);
+    return hipGetLastError();
+}
+
+hipError_t launch_mlp_backward(hipStream_t st, const MlpWork& w, uint64_t P, uint32_t rows_per_block,
+                               uint32_t nblocks)
+{
+    if (P == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_mlp_backward_rows, dim3(blocks256(P)), dim3(256), 0, st, w.params, w.Z1, w.Z2, w.pt, w.dpred,
+                       P, w.dZ1, w.dZ2);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_mlp_wgrad_partial, dim3(nblocks), dim3(256), 0, st, w.X, w.Z1, w.Z2, w.dZ1, w.dZ2, w.pt,
+                       w.dpred, P, rows_per_block, w.partial);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_mlp_wgrad_reduce, dim3(blocks256(kMlpParams)), dim3(256), 0, st, w.partial, nblocks, w.grad);
+    return hipGetLastError();
+}
+
+hipError_t launch_mlp_adam(hipStream_t st, const MlpWork& w, float lr, float beta1, float beta2, float eps, float bc1,
+                           float bc2)
+{
+    hipLaunchKernelGGL(k_mlp_adam, dim3(blocks256(kMlpParams)), dim3(256), 0, st, w.params, w.grad, w.adam_m, w.adam_v,
+                       lr, beta1, beta2, eps, bc1, bc2);
+    return hipGetLastError();
+}
+
+hipError_t launch_train_targets(hipStream_t st, const DeviceState& d, int nscenes, const MlpWork& w,
+                                const sph_particle_t* later, uint32_t first, uint32_t stride, uint32_t T)
+{
+    if (T == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_train_targets, dim3(blocks256(T)), dim3(256), 0, st, d.scenes, nscenes, d.perm, d.aos, later,
+                       first, stride, T, 1.0f / w.norm.acc_scale, w.tgt, w.wt);
+    return hipGetLastError();
+}
+
+hipError_t launch_coarse_integrate(hipStream_t st, const DeviceState& d, const MlpWork& w, bool use_force)
+{
+    hipLaunchKernelGGL(k_coarse_integrate, dim3(d.ntiles), dim3(kTile), 0, st, d.tiles, d.scenes, d.PA, d.VA, d.rho,
+                       d.D2, w.pred, d.perm, d.aos, d.PB, d.VB, w.norm.acc_scale, use_force ? 1 : 0);
+    return hipGetLastError();
+}
+
+hipError_t launch_coarse_mlp(hipStream_t st, const DeviceState& d, const MlpWork& w, int path)
+{
+    (void)path;
+    const int sub = d.tuning.cells_per_h;
+    if (sub == 1)
+        hipLaunchKernelGGL(k_coarse_mlp_fp32<1>, dim3(d.ntiles), dim3(256), 0, st, d.tiles, d.scenes, d.start, d.PA,
+                           d.VA, d.rho, d.D2, w.params, w.norm, w.pred);
+    else if (sub == 2)
+        hipLaunchKernelGGL(k_coarse_mlp_fp32<2>, dim3(d.ntiles), dim3(256), 0, st, d.tiles, d.scenes, d.start, d.PA,
+                           d.VA, d.rho, d.D2, w.params, w.norm, w.pred);
+    else
+        hipLaunchKernelGGL(k_coarse_mlp_fp32<3>, dim3(d.ntiles), dim3(256), 0, st, d.tiles, d.scenes, d.start, d.PA,
+                           d.VA, d.rho, d.D2, w.params, w.norm, w.pred);
+    return hipGetLastError();
+}
+
+hipError_t launch_pred_to_aos(hipStream_t st, const DeviceState& d, const MlpWork& w, float* out)
+{
+    hipLaunchKernelGGL(k_pred_to_aos, dim3(blocks256(d.n)), dim3(256), 0, st, w.pred, d.perm, d.n, out);
+    return hipGetLastError();
+}
+
+}  // namespace nprsph

@@ -32,7 +32,10 @@ EXPORTS = [
     "sph_upload", "sph_download", "sph_init_lattice", "sph_rho_pres", "sph_force", "sph_integrate", "sph_step",
     "sph_device_buffer", "sph_stream", "sph_synchronize", "sph_grid_info", "sph_total_cells", "sph_download_grid",
     "sph_metrics", "sph_set_timing", "sph_kernel_times", "sph_get_tuning", "sph_set_tuning", "sph_autotune",
+    "sph_mlp_set", "sph_mlp_get", "sph_mlp_train", "sph_mlp_predict", "sph_coarse_step", "sph_copy_state",
 ]
+
+MLP_PARAMS = 4995
 
 
 class SphError(RuntimeError):
@@ -70,6 +73,32 @@ class GridInfo(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("cells_per_h", C.c_int32), ("prune", C.c_int32), ("block", C.c_int32), ("blocks_per_xcd", C.c_int32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class MlpNorm(C.Structure):
+    _fields_ = [("v_scale", C.c_float), ("rho_scale", C.c_float), ("pi_scale", C.c_float), ("acc_scale", C.c_float)]
+
+
+class TrainOpts(C.Structure):
+    _fields_ = [("first", C.c_uint32), ("stride", C.c_uint32), ("lr", C.c_float), ("beta1", C.c_float),
+                ("beta2", C.c_float), ("eps", C.c_float), ("update", C.c_int32)]
+
+
+class TrainResult(C.Structure):
+    _fields_ = [("loss", C.c_double), ("pairs", C.c_uint64), ("targets", C.c_uint32), ("weighted", C.c_uint32)]
+
+
+class CoarseOpts(C.Structure):
+    _fields_ = [("use_network", C.c_int32), ("judge_every", C.c_int32), ("drift_limit", C.c_float),
+                ("cooldown", C.c_int32)]
+
+
+class CoarseStats(C.Structure):
+    _fields_ = [("coarse_steps", C.c_int64), ("plain_steps", C.c_int64), ("judged", C.c_int64),
+                ("fallbacks", C.c_int64), ("last_drift", C.c_double), ("max_drift", C.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -130,6 +159,12 @@ def load_library(path: str | None = None):
         "sph_get_tuning": (C.c_int, [vp, C.POINTER(Tuning)]),
         "sph_set_tuning": (C.c_int, [vp, C.POINTER(Tuning)]),
         "sph_autotune": (C.c_int, [vp, C.c_int, C.POINTER(Tuning), C.POINTER(C.c_double)]),
+        "sph_mlp_set": (C.c_int, [vp, vp, C.c_size_t, C.POINTER(MlpNorm)]),
+        "sph_mlp_get": (C.c_int, [vp, vp, C.c_size_t, C.POINTER(MlpNorm)]),
+        "sph_mlp_train": (C.c_int, [vp, vp, C.POINTER(TrainOpts), vp, C.POINTER(TrainResult)]),
+        "sph_mlp_predict": (C.c_int, [vp, C.c_int, vp]),
+        "sph_coarse_step": (C.c_int, [vp, C.c_int, C.POINTER(CoarseOpts), C.POINTER(CoarseStats)]),
+        "sph_copy_state": (C.c_int, [vp, vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -307,6 +342,40 @@ class Simulation:
         t, ms = Tuning(), C.c_double()
         self._check(self._L.sph_autotune(self._h, int(trial_steps), C.byref(t), C.byref(ms)))
         return t, ms.value
+
+    # -- learned coarse step -------------------------------------------------
+    def mlp_set(self, params, norm: MlpNorm):
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        self._check(self._L.sph_mlp_set(self._h, p.ctypes.data, p.size, C.byref(norm)))
+
+    def mlp_get(self):
+        p = np.empty(MLP_PARAMS, dtype=np.float32)
+        nm = MlpNorm()
+        self._check(self._L.sph_mlp_get(self._h, p.ctypes.data, p.size, C.byref(nm)))
+        return p, nm
+
+    def mlp_train(self, later: "Simulation", first=0, stride=1, lr=1e-3, beta1=0.9, beta2=0.999, eps=1e-8,
+                  update=True, want_grad=False):
+        o = TrainOpts(first, stride, lr, beta1, beta2, eps, 1 if update else 0)
+        r = TrainResult()
+        g = np.empty(MLP_PARAMS, dtype=np.float32) if want_grad else None
+        self._check(self._L.sph_mlp_train(self._h, later.handle, C.byref(o), None if g is None else g.ctypes.data,
+                                          C.byref(r)))
+        return (r, g) if want_grad else r
+
+    def mlp_predict(self, path: int = 0) -> np.ndarray:
+        out = np.empty((self.n, 3), dtype=np.float32)
+        self._check(self._L.sph_mlp_predict(self._h, int(path), out.ctypes.data))
+        return out
+
+    def coarse_step(self, n=1, use_network=True, judge_every=0, drift_limit=10.0, cooldown=0) -> CoarseStats:
+        o = CoarseOpts(1 if use_network else 0, judge_every, drift_limit, cooldown)
+        st = CoarseStats()
+        self._check(self._L.sph_coarse_step(self._h, int(n), C.byref(o), C.byref(st)))
+        return st
+
+    def copy_state_from(self, other: "Simulation"):
+        self._check(self._L.sph_copy_state(self._h, other.handle))
 
     def set_timing(self, on: bool):
         self._check(self._L.sph_set_timing(self._h, 1 if on else 0))

@@ -1,0 +1,205 @@
+"""GPU tests of the learned coarse step through the C ABI, against the FP64
+reference in oracle/surrogate_ref.py.
+
+* sph_mlp_train gradients (FP32 forward/backward kernels, deterministic
+  reduction) equal the FP64 analytic gradient of the same batch to 1e-4
+  relative norm; the loss to 1e-5.
+* sph_mlp_predict (FP32 path) equals the FP64 forward to 1e-4.
+* Adam steps on the GPU lower the loss of a fixed batch.
+* One coarse step (network, and the SPH force held for 4 fine steps) equals
+  the FP64 coarse integrator from the same state.
+* The judge with a zero drift limit keeps the 4-plain-step result.
+"""
+import importlib.util
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+@pytest.fixture(scope="module")
+def ref():
+    spec = importlib.util.spec_from_file_location("surrogate_ref", os.path.join(ROOT, "oracle", "surrogate_ref.py"))
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    return mod
+
+
+NORM = dict(v_scale=0.1, rho_scale=1e-5, pi_scale=5e-4, acc_scale=2e4)
+
+
+def _norm(pkg):
+    return pkg.sph.MlpNorm(NORM["v_scale"], NORM["rho_scale"], NORM["pi_scale"], NORM["acc_scale"])
+
+
+def _scene(pkg, orc, n=1500, steps=30):
+    sc = pkg.scenes.dam10k()
+    p0 = orc.lattice(sc.nx, sc.ny, sc.nz, sc.spacing, sc.origin, sc.jitter, sc.seed, count=sc.count)[:n].copy()
+    b = pkg.sph.make_boundary(sc.lower, sc.upper)
+    sim = pkg.Simulation(n, boundaries=b)
+    sim.upload(p0)
+    sim.step(steps)
+    return sim, sc
+
+
+def _cpu_pairs(state, perm, h, slots):
+    """Neighbour pairs (r < h, j != i) of the given target slots, by brute force
+    in float32 arithmetic on the same positions."""
+    pos = state["pos"][perm, :3]
+    rows_t, rows_j = [], []
+    h2 = np.float32(h) * np.float32(h)
+    margin = []
+    for t, s in enumerate(slots):
+        d = (pos[s][None, :] - pos).astype(np.float32)
+        r2 = (d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1] + d[:, 2] * d[:, 2]).astype(np.float32)
+        near = np.nonzero((r2 < h2) & (np.arange(len(pos)) != s))[0]
+        margin.append(np.min(np.abs(r2.astype(np.float64) / float(h2) - 1.0)))
+        rows_t += [t] * len(near)
+        rows_j += list(near)
+    return np.array(rows_t), np.array(rows_j), float(np.min(margin))
+
+
+def _features(ref, state, perm, slots, rows_t, rows_j, h):
+    st = state[perm]
+    x = st["pos"][:, :3].astype(np.float64)
+    v = st["vel"][:, :3].astype(np.float64)
+    rho = st["extras"][:, 0].astype(np.float64)
+    p = st["extras"][:, 1].astype(np.float64)
+    inv_rho = (np.float32(1.0) / st["extras"][:, 0]).astype(np.float64)
+    i = np.asarray(slots)[rows_t]
+    j = rows_j
+    return ref.pair_features(x[i], x[j], v[i], v[j], rho[i], rho[j], p[i], p[j], inv_rho[j], float(h), NORM)
+
+
+def _targets(state_now, state_later, perm, slots, sc):
+    a = state_now[perm][slots]
+    b = state_later[perm][slots]
+    dt = np.float32(1.0) / np.float32(10000.0)
+    g = np.array([0.0, np.float32(-9806.65), 0.0])
+    acc = (b["vel"][:, :3].astype(np.float64) - a["vel"][:, :3]) / (4 * float(dt)) - g
+    lo, hi = np.array(sc.lower, dtype=np.float32), np.array(sc.upper, dtype=np.float32)
+    ok = np.ones(len(slots), dtype=bool)
+    for st in (a, b):
+        ps = st["pos"][:, :3]
+        ok &= np.all(np.isfinite(ps), axis=1) & np.all((np.abs(ps - lo) > 1e-6) & (np.abs(ps - hi) > 1e-6), axis=1)
+    tgt = np.where(ok[:, None], acc / NORM["acc_scale"], 0.0)
+    return tgt, ok.astype(np.float64)
+
+
+def test_training_gradient_matches_fp64(pkg, orc, ref, gpu_lib):
+    sim, sc = _scene(pkg, orc)
+    later = pkg.Simulation(sim.n, boundaries=pkg.sph.make_boundary(sc.lower, sc.upper))
+    later.copy_state_from(sim)
+    later.step(4)
+    params = ref.init_params(7)
+    params[ref.NPARAMS - 195:] += np.random.default_rng(8).normal(0, 0.05, 195).astype(np.float32)
+    sim.mlp_set(params, _norm(pkg))
+    first, stride = 1, 2
+    res, grad = sim.mlp_train(later, first=first, stride=stride, update=False, want_grad=True)
+    now = sim.download()
+    lat = later.download()
+    _, perm = sim.download_grid()
+    slots = np.arange(first, sim.n, stride)
+    h = orc.smoothing_length()
+    rows_t, rows_j, margin = _cpu_pairs(now, perm, h, slots)
+    assert margin > 1e-6, "fixture has a pair on the r = h boundary"
+    assert res.pairs == len(rows_t) and res.targets == len(slots)
+    f = _features(ref, now, perm, slots, rows_t, rows_j, h)
+    tgt, w = _targets(now, lat, perm, slots, sc)
+    assert res.weighted == int(w.sum())
+    L, g, _ = ref.loss_and_grad(params.astype(np.float64), f, rows_t, tgt, w)
+    assert res.loss == pytest.approx(L, rel=1e-5)
+    assert np.linalg.norm(grad - g) <= 1e-4 * np.linalg.norm(g)
+    sim.close()
+    later.close()
+
+
+def test_predict_fp32_matches_fp64(pkg, orc, ref, gpu_lib):
+    sim, sc = _scene(pkg, orc, n=1200, steps=20)
+    params = ref.init_params(11)
+    sim.mlp_set(params, _norm(pkg))
+    pred = sim.mlp_predict(path=0)
+    now = sim.download()
+    _, perm = sim.download_grid()
+    slots = np.arange(sim.n)
+    h = orc.smoothing_length()
+    rows_t, rows_j, margin = _cpu_pairs(now, perm, h, slots)
+    assert margin > 1e-6
+    f = _features(ref, now, perm, slots, rows_t, rows_j, h)
+    m, _ = ref.forward(params.astype(np.float64), f)
+    cpu = np.zeros((sim.n, 3))
+    np.add.at(cpu, rows_t, m)
+    cpu_aos = np.zeros_like(cpu)
+    cpu_aos[perm] = cpu
+    scale = np.abs(m).sum() / len(m) * 200
+    assert np.max(np.abs(pred - cpu_aos)) <= 1e-4 * scale
+    sim.close()
+
+
+def test_adam_lowers_loss(pkg, orc, ref, gpu_lib):
+    sim, sc = _scene(pkg, orc, n=1500, steps=40)
+    later = pkg.Simulation(sim.n, boundaries=pkg.sph.make_boundary(sc.lower, sc.upper))
+    later.copy_state_from(sim)
+    later.step(4)
+    sim.mlp_set(ref.init_params(5), _norm(pkg))
+    losses = [sim.mlp_train(later, lr=3e-3).loss for _ in range(60)]
+    assert losses[-1] < 0.5 * losses[0]
+    p, nm = sim.mlp_get()
+    assert np.all(np.isfinite(p)) and nm.acc_scale == NORM["acc_scale"]
+    sim.close()
+    later.close()
+
+
+@pytest.mark.parametrize("use_network", [False, True])
+def test_coarse_step_matches_fp64_integrator(pkg, orc, ref, gpu_lib, use_network):
+    sim, sc = _scene(pkg, orc, n=2000, steps=25)
+    params = ref.init_params(13)
+    sim.mlp_set(params, _norm(pkg))
+    s0 = sim.download()
+    if use_network:
+        pred = sim.mlp_predict(path=0).astype(np.float64)
+        acc = NORM["acc_scale"] * pred + np.array([0.0, np.float32(-9806.65), 0.0])
+    else:
+        sim.rho_pres()
+        sim.force()
+        sf = sim.download()
+        acc = sf["force"][:, :3].astype(np.float64) / sf["extras"][:, [0]].astype(np.float64)
+    sim.upload(s0)
+    st = sim.coarse_step(1, use_network=use_network)
+    assert st.coarse_steps == 1 and st.plain_steps == 0
+    got = sim.download()
+    dt = float(np.float32(1.0) / np.float32(10000.0))
+    x, v = ref.coarse_integrate(s0["pos"][:, :3].astype(np.float64), s0["vel"][:, :3].astype(np.float64), acc,
+                                sc.lower, sc.upper, dt, float(np.float32(0.3)))
+    vscale = np.abs(s0["vel"][:, :3]) + 4 * dt * np.abs(acc) + 1e-3
+    # Axes whose unclamped FP64 position lies within 1e-5 of a wall may take
+    # either clamp branch in float32.
+    xu = s0["pos"][:, :3].astype(np.float64) + 4 * dt * s0["vel"][:, :3] + 10 * dt * dt * acc
+    lo, hi = np.array(sc.lower)[None, :], np.array(sc.upper)[None, :]
+    strict = (np.abs(xu - lo) > 1e-5) & (np.abs(xu - hi) > 1e-5)
+    assert strict.mean() > 0.9
+    assert np.all((np.abs(got["vel"][:, :3] - v) <= 2e-5 * vscale)[strict])
+    assert np.all((np.abs(got["pos"][:, :3] - x) <= 1e-6)[strict])
+    sim.close()
+
+
+def test_judge_zero_limit_keeps_plain_result(pkg, orc, ref, gpu_lib):
+    sim, sc = _scene(pkg, orc, n=2000, steps=10)
+    sim.mlp_set(ref.init_params(17), _norm(pkg))
+    s0 = sim.download()
+    st = sim.coarse_step(3, use_network=True, judge_every=1, drift_limit=0.0, cooldown=0)
+    assert st.judged == 3 and st.fallbacks == 3 and st.coarse_steps == 0 and st.plain_steps == 12
+    got = sim.download()
+    with pkg.Simulation(sim.n, boundaries=pkg.sph.make_boundary(sc.lower, sc.upper)) as plain:
+        plain.upload(s0)
+        plain.step(12)
+        want = plain.download()
+    assert got["pos"].tobytes() == want["pos"].tobytes() and got["vel"].tobytes() == want["vel"].tobytes()
+    # cooldown: after a fallback the next coarse steps run plain.
+    sim.upload(s0)
+    st = sim.coarse_step(4, use_network=True, judge_every=4, drift_limit=0.0, cooldown=3)
+    assert st.fallbacks == 1 and st.plain_steps == 16 and st.coarse_steps == 0
+    sim.close()

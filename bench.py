@@ -77,6 +77,80 @@ def cpu_baseline(pkg, scene, budget_s: float):
                       f"step, {threads} OpenMP threads ({el:.1f} s)"}
 
 
+def density_timeline(sim, total_fine, every_fine, coarse=None):
+    """Mean density (scene 0) every `every_fine` fine steps of simulated time,
+    advancing with plain steps or with coarse steps (dict of coarse options)."""
+    out = []
+    for _ in range(total_fine // every_fine):
+        if coarse is None:
+            sim.step(every_fine)
+        else:
+            sim.coarse_step(every_fine // 4, **coarse)
+        out.append(sim.metrics(0).mean_rho)
+    return np.array(out)
+
+
+def learned_form(pkg, scenes, warmup, fine_steps, plain_ms_per_step, weights_path=None):
+    """Learned coarse step on the same workload: timed run of fine_steps/4
+    coarse steps (judge every 50, drift limit 0.01*rho0), then untimed
+    mean-density timelines of plain, learned and no-network coarse runs from
+    the same warmed state."""
+    sys.path.insert(0, os.path.join(ROOT, "npr-sph_amd"))
+    import train_surrogate
+
+    params, norm, meta = train_surrogate.load_weights(weights_path)
+    ncoarse = fine_steps // 4
+    rho0 = 1000.0
+    judge = dict(use_network=True, judge_every=50, drift_limit=0.01 * rho0, cooldown=0)
+    sim = make_sim(pkg, scenes)
+    sim.step(warmup)
+    sim.mlp_set(params, norm)
+    s0 = sim.download()
+    sim.coarse_step(1, **judge)  # kernel warm-up
+    sim.upload(s0)
+    sim.synchronize()
+    t0 = time.perf_counter()
+    st = sim.coarse_step(ncoarse, **judge)
+    sim.synchronize()
+    el = time.perf_counter() - t0
+    n = sum(sc.count for sc in scenes)
+    ms_per_coarse = 1e3 * el / ncoarse
+    # accuracy: mean density every 20 fine steps of simulated time
+    every = 20
+    sim.upload(s0)
+    plain = density_timeline(sim, fine_steps, every)
+    sim.upload(s0)
+    learned = density_timeline(sim, fine_steps, every, coarse=judge)
+    sim.upload(s0)
+    ablation = density_timeline(sim, fine_steps, every,
+                                coarse=dict(use_network=False, judge_every=0, drift_limit=0.0, cooldown=0))
+    sim.close()
+
+    def err(x):
+        ok = np.isfinite(x)
+        if not ok.all():
+            return None
+        return float(np.mean(np.abs(x - plain) / plain))
+
+    plain_ms_per_4 = 4.0 * plain_ms_per_step
+    return {
+        "value": n * 4 * ncoarse / el,
+        "unit": "particle-steps/s (fine-step equivalents)",
+        "ms_per_coarse_step": ms_per_coarse,
+        "sim_seconds_per_wall_second": 4 * 1e-4 * ncoarse / el,
+        "plain_sim_seconds_per_wall_second": 1e-4 / (plain_ms_per_step * 1e-3),
+        "speedup_vs_plain": plain_ms_per_4 / ms_per_coarse,
+        "coarse_steps": ncoarse,
+        "judge": st.as_dict(),
+        "drift_definition": "judge every 50 coarse steps: |mean rho(1 coarse step) - mean rho(4 plain steps)| from the "
+                            "same state; above 0.01*rho0 the plain result is kept",
+        "mean_density_rel_error_vs_plain": err(learned),
+        "no_network_mean_density_rel_error_vs_plain": err(ablation),
+        "timeline_every_fine_steps": every,
+        "weights": {"rollouts": meta.get("rollouts"), "updates": meta.get("updates"), "loss_last20": meta.get("loss_last20")},
+    }
+
+
 def measured_traffic(workload, kernel, tuning):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE /
     WRITE_SIZE summary (calibrated by a plain copy), when it was measured for
@@ -118,6 +192,7 @@ def main(argv=None):
     ap.add_argument("--tune", default="auto",
                     help="'auto' (autotune on the warmed state, untimed), 'default', or cells_per_h,prune,block,bpx")
     ap.add_argument("--tune-steps", type=int, default=5)
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned coarse-step measurement")
     args = ap.parse_args(argv)
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (each GPU host here has one GPU, "
@@ -166,6 +241,10 @@ def main(argv=None):
     m = sim.metrics(0)
     sim.close()
 
+    learned = None
+    if not args.no_learned and len(scenes) == 1:
+        learned = learned_form(pkg, scenes, args.warmup, args.steps, ms_per_step)
+
     cpu = None
     if args.cpu_budget > 0:
         cpu = cpu_baseline(pkg, scenes[0], args.cpu_budget)
@@ -192,6 +271,7 @@ def main(argv=None):
         "roofline": roofline,
         "kernels": kern,
         "cpu_baseline": cpu,
+        "learned": learned,
         "state_after": {"mean_rho_scene0": m.mean_rho, "max_speed_scene0": m.max_speed, "nonfinite": m.nonfinite},
     }
     print(json.dumps(out))

@@ -409,7 +409,7 @@ void free_all(sph_ctx* c)
                     c->d.slot, c->d.torig, c->d.tsrc, c->d.tkey, c->d.count, c->d.start, c->d.bsum, c->d.partials,
                     c->d_scenes, c->d_tiles, c->d_tiles128, c->d_backup, c->d_judge_s, c->d_judge_p, c->d_pred_aos,
                     w.params, w.adam_m, w.adam_v, w.grad, w.cnt, w.off, w.tgt, w.wt, w.pred, w.dpred, w.lossp,
-                    w.pj, w.pt, w.X, w.Z1, w.Z2, w.dZ1, w.dZ2, w.M, w.partial};
+                    w.pj, w.pt, w.X, w.Z1, w.Z2, w.dZ1, w.dZ2, w.M, w.partial, w.pack_a, w.pack_c};
     for (void* p : bufs)
         if (p) (void)hipFree(p);
     if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
@@ -439,7 +439,8 @@ int ensure_mlp_base(sph_ctx* c)
     MlpWork& w = c->mlp;
     if (w.params) return SPH_OK;
     if (dalloc(&w.params, kMlpParams) != hipSuccess || dalloc(&w.adam_m, kMlpParams) != hipSuccess ||
-        dalloc(&w.adam_v, kMlpParams) != hipSuccess || dalloc(&w.grad, kMlpParams) != hipSuccess) {
+        dalloc(&w.adam_v, kMlpParams) != hipSuccess || dalloc(&w.grad, kMlpParams) != hipSuccess ||
+        dalloc(&w.pack_a, kMlpPackA) != hipSuccess || dalloc(&w.pack_c, kMlpPackC) != hipSuccess) {
         (void)hipGetLastError();
         return fail(c, SPH_ERR_NOMEM, "cannot allocate network parameters");
     }
@@ -1019,6 +1020,7 @@ int sph_mlp_set(sph_ctx* c, const float* params, size_t n, const sph_mlp_norm_t*
     SPH_HIP(c, hipMemsetAsync(c->mlp.adam_v, 0, sizeof(float) * kMlpParams, c->stream));
     SPH_HIP(c, hipStreamSynchronize(c->stream));
     c->mlp.norm = MlpNorm{norm->v_scale, norm->rho_scale, norm->pi_scale, norm->acc_scale};
+    c->mlp.pack_stale = true;
     c->mlp_loaded = true;
     c->adam_step = 0;
     return SPH_OK;
@@ -1104,6 +1106,7 @@ int sph_mlp_train(sph_ctx* c, const sph_ctx* later, const sph_train_opts_t* o, f
         const float bc1 = 1.f - std::pow(o->beta1, (float)c->adam_step);
         const float bc2 = 1.f - std::pow(o->beta2, (float)c->adam_step);
         SPH_HIP(c, launch_mlp_adam(c->stream, w, o->lr, o->beta1, o->beta2, o->eps, bc1, bc2));
+        w.pack_stale = true;
     }
     SPH_HIP(c, hipStreamSynchronize(c->stream));
     double L = 0.0;
@@ -1132,7 +1135,11 @@ int sph_mlp_predict(sph_ctx* c, int path, float* out)
         c->d_pred_aos = nullptr;
         return fail(c, SPH_ERR_NOMEM, "cannot allocate prediction buffer");
     }
-    SPH_RUN(c, "mlp_messages", launch_coarse_mlp(c->stream, c->d, c->mlp, path));
+    if (path == 1 && c->mlp.pack_stale) {
+        SPH_RUN(c, "mlp_pack", launch_mlp_pack(c->stream, c->mlp));
+        c->mlp.pack_stale = false;
+    }
+    SPH_RUN(c, "mlp_messages", launch_coarse_mlp(c->stream, c->d, c->nscenes, c->mlp, path));
     SPH_HIP(c, launch_pred_to_aos(c->stream, c->d, c->mlp, c->d_pred_aos));
     SPH_HIP(c, hipMemcpyAsync(out, c->d_pred_aos, sizeof(float) * 3u * c->n, hipMemcpyDeviceToHost, c->stream));
     SPH_HIP(c, hipStreamSynchronize(c->stream));
@@ -1160,7 +1167,11 @@ int coarse_once(sph_ctx* c, bool use_network)
     int rc;
     if ((rc = density_pass(c, false)) != SPH_OK) return rc;
     if (use_network) {
-        SPH_RUN(c, "mlp_messages", launch_coarse_mlp(c->stream, c->d, c->mlp, 1));
+        if (c->mlp.pack_stale) {
+            SPH_RUN(c, "mlp_pack", launch_mlp_pack(c->stream, c->mlp));
+            c->mlp.pack_stale = false;
+        }
+        SPH_RUN(c, "mlp_messages", launch_coarse_mlp(c->stream, c->d, c->nscenes, c->mlp, 1));
     } else {
         SPH_RUN(c, "force", launch_force(c->stream, c->d, false));
     }

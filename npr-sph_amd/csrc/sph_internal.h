@@ -94,7 +94,13 @@ struct MlpWork {
     float* dZ2 = nullptr;
     float* M = nullptr;        // 3 per pair
     float* partial = nullptr;  // kMlpParams per gradient block
+    __bf16* pack_a = nullptr;  // MFMA operand images (k_mlp_pack)
+    float* pack_c = nullptr;
+    bool pack_stale = true;    // params changed since the last pack
 };
+
+constexpr int kMlpPackA = 4 * 64 * 8 + 4 * 2 * 64 * 8 + 2 * 64 * 8;  // bf16 entries
+constexpr int kMlpPackC = 4 * 64 * 4 * 2 + 64 * 4;                  // float entries
 
 // Each launcher enqueues one kernel on `s` and returns hipGetLastError().
 hipError_t launch_set_scene(hipStream_t s, SceneDev* dst, const SceneDev& value);
@@ -124,7 +130,8 @@ hipError_t launch_mlp_adam(hipStream_t s, const MlpWork& w, float lr, float beta
 hipError_t launch_train_targets(hipStream_t s, const DeviceState& d, int nscenes, const MlpWork& w,
                                 const sph_particle_t* later, uint32_t first, uint32_t stride, uint32_t T);
 hipError_t launch_coarse_integrate(hipStream_t s, const DeviceState& d, const MlpWork& w, bool use_force);
-hipError_t launch_coarse_mlp(hipStream_t s, const DeviceState& d, const MlpWork& w, int path);
+hipError_t launch_mlp_pack(hipStream_t s, const MlpWork& w);
+hipError_t launch_coarse_mlp(hipStream_t s, const DeviceState& d, int nscenes, const MlpWork& w, int path);
 hipError_t launch_pred_to_aos(hipStream_t s, const DeviceState& d, const MlpWork& w, float* out);
 
 }  // namespace nprsph

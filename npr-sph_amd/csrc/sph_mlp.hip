@@ -14,6 +14,8 @@
 //                                 added in a fixed order (deterministic)
 //   k_mlp_adam                    Adam update of the float32 master weights
 // The coarse step integrates with k_coarse_integrate.
+#include <algorithm>
+
 #include "sph_gather.h"
 #include "sph_internal.h"
 
@@ -35,6 +37,12 @@ static_assert(kOffB3 + kNout == kMlpParams, "parameter layout");
 constexpr int kRowX = 0, kRowOne = 9, kRowH1 = 10, kRowH2 = kRowH1 + kNh, kRowDz1 = kRowH2 + kNh,
               kRowDz2 = kRowDz1 + kNh, kRowDm = kRowDz2 + kNh, kRowLen = kRowDm + kNout;
 constexpr int kGradRows = 16;  // rows staged in LDS per sub-chunk
+
+// MlpPackLayout: offsets of the per-lane MFMA operand images (bf16 for A,
+// float32 for the bias-initialised accumulators C).
+constexpr int kPackA1 = 0, kPackA2 = kPackA1 + 4 * 64 * 8, kPackA3 = kPackA2 + 4 * 2 * 64 * 8,
+              kPackAEnd = kPackA3 + 2 * 64 * 8;
+constexpr int kPackC1 = 0, kPackC2 = kPackC1 + 4 * 64 * 4, kPackC3 = kPackC2 + 4 * 64 * 4, kPackCEnd = kPackC3 + 64 * 4;
 
 __device__ __forceinline__ uint32_t scene_of(const SceneDev* __restrict__ scenes, int nscenes, uint32_t s)
 {
@@ -478,6 +486,232 @@ __global__ __launch_bounds__(256) void k_coarse_mlp_fp32(const Tile* __restrict_
     for (int c = 0; c < kNout; ++c) pred[s * 3 + c] = acc[c];
 }
 
+// ---- BF16 MFMA inference --------------------------------------------------
+//
+// v_mfma_f32_16x16x32_bf16 lane maps (lane l, element e):
+//   A (16 rows x 32 k):  row l%16, k 8*(l/16)+e
+//   B (32 k x 16 cols):  col l%16, k 8*(l/16)+e
+//   C/D (16 x 16 fp32):  row 4*(l/16)+e, col l%16
+// The network runs transposed, hidden units x pairs: H1' = W1' X' + b1,
+// H2' = W2' g(H1') + b2, M' = W3' g(H2') + b3, 16 pairs per tile.  A layer's
+// D registers hold, per lane, pair l%16 and hidden units 16m + 4(l/16) + e
+// (m = 0..3 tiles).  Those are exactly the B elements of k-step s of the next
+// layer if its K order is permuted as
+//   k-slot 32s + 8g + 4mm + e  <->  hidden unit 16(2s+mm) + 4g + e,
+// so layers chain in registers; the permutation is folded into the packed
+// weights (k_mlp_pack).
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__host__ __device__ __forceinline__ int kslot_unit(int kslot)
+{
+    const int s = kslot / 32, g = (kslot % 32) / 8, mm = (kslot % 8) / 4, e = kslot % 4;
+    return 16 * (2 * s + mm) + 4 * g + e;
+}
+
+// Per-lane operand images of the network (see MlpPackLayout).
+__global__ __launch_bounds__(256) void k_mlp_pack(const float* __restrict__ params, __bf16* __restrict__ pa,
+                                                  float* __restrict__ pc)
+{
+    // A1[m][l][e]: W1[k][16m + l%16], k = 8(l/16)+e < 9
+    for (int i = threadIdx.x; i < 4 * 64 * 8; i += 256) {
+        const int m = i / 512, l = (i / 8) % 64, e = i % 8, k = 8 * (l / 16) + e;
+        pa[kPackA1 + i] = (__bf16)(k < kNin ? params[kOffW1 + k * kNh + 16 * m + l % 16] : 0.f);
+    }
+    // A2[m][s][l][e]: W2[unit(32s + 8(l/16) + e)][16m + l%16]
+    for (int i = threadIdx.x; i < 4 * 2 * 64 * 8; i += 256) {
+        const int m = i / 1024, s = (i / 512) % 2, l = (i / 8) % 64, e = i % 8;
+        const int u = kslot_unit(32 * s + 8 * (l / 16) + e);
+        pa[kPackA2 + i] = (__bf16)params[kOffW2 + u * kNh + 16 * m + l % 16];
+    }
+    // A3[s][l][e]: W3[unit(32s + 8(l/16) + e)][l%16] for l%16 < 3
+    for (int i = threadIdx.x; i < 2 * 64 * 8; i += 256) {
+        const int s = i / 512, l = (i / 8) % 64, e = i % 8;
+        const int u = kslot_unit(32 * s + 8 * (l / 16) + e);
+        pa[kPackA3 + i] = (__bf16)(l % 16 < kNout ? params[kOffW3 + u * kNout + l % 16] : 0.f);
+    }
+    // C1[m][l][e] = b1[16m + 4(l/16) + e], C2 likewise with b2, C3[l][e] = b3[4(l/16)+e] (rows < 3)
+    for (int i = threadIdx.x; i < 4 * 64 * 4; i += 256) {
+        const int m = i / 256, l = (i / 4) % 64, e = i % 4, u = 16 * m + 4 * (l / 16) + e;
+        pc[kPackC1 + i] = params[kOffB1 + u];
+        pc[kPackC2 + i] = params[kOffB2 + u];
+    }
+    for (int i = threadIdx.x; i < 64 * 4; i += 256) {
+        const int l = i / 4, e = i % 4, row = 4 * (l / 16) + e;
+        pc[kPackC3 + i] = row < kNout ? params[kOffB3 + row] : 0.f;
+    }
+}
+
+constexpr int kHitCap = 512;  // hit list entries per wave
+
+// Summed messages of every slot on MFMA: one wave per target particle.  The
+// wave scans the target's stencil rows 64 candidates at a time, compacts the
+// hits into an LDS list, and runs the network on 16-pair tiles.
+template <int SUB>
+__global__ __launch_bounds__(256) void k_coarse_mlp_mfma(const SceneDev* __restrict__ scenes, int nscenes,
+                                                         const uint32_t* __restrict__ start,
+                                                         const float4* __restrict__ P, const float4* __restrict__ V,
+                                                         const float* __restrict__ rho, const float2* __restrict__ D2,
+                                                         const __bf16* __restrict__ pa, const float* __restrict__ pc,
+                                                         MlpNorm nm, uint32_t n, float* __restrict__ pred)
+{
+    __shared__ uint32_t hits_all[4][kHitCap];
+    const int lane = (int)(threadIdx.x & 63);
+    const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    uint32_t* hits = hits_all[wv];
+    bf16x8 a1[4], a2[4][2], a3[2];
+    f32x4 c1[4], c2[4], c3;
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+#pragma unroll
+        for (int e = 0; e < 8; ++e) a1[m][e] = pa[kPackA1 + (m * 64 + lane) * 8 + e];
+#pragma unroll
+        for (int s = 0; s < 2; ++s)
+#pragma unroll
+            for (int e = 0; e < 8; ++e) a2[m][s][e] = pa[kPackA2 + ((m * 2 + s) * 64 + lane) * 8 + e];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            c1[m][e] = pc[kPackC1 + (m * 64 + lane) * 4 + e];
+            c2[m][e] = pc[kPackC2 + (m * 64 + lane) * 4 + e];
+        }
+    }
+#pragma unroll
+    for (int s = 0; s < 2; ++s)
+#pragma unroll
+        for (int e = 0; e < 8; ++e) a3[s][e] = pa[kPackA3 + (s * 64 + lane) * 8 + e];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) c3[e] = pc[kPackC3 + lane * 4 + e];
+
+    const uint32_t nwaves = gridDim.x * 4u;
+    for (uint32_t si = blockIdx.x * 4u + (uint32_t)wv; si < n; si += nwaves) {
+        const uint32_t s = (uint32_t)__builtin_amdgcn_readfirstlane((int)si);
+        const SceneDev& sc = scenes[scene_of(scenes, nscenes, s)];
+        const float4 xi = P[s], vi = V[s];
+        const float rhoi = rho[s], pi = D2[s].x;
+        const float h2 = sc.h2, inv_h = 1.0f / sc.h;
+        float acc0 = 0.f, acc1 = 0.f, acc2 = 0.f;
+        int nh = 0;
+
+        // Run the network on the first `count` listed hits in 16-pair tiles
+        // (all lanes take part); with `partial`, a last incomplete tile too.
+        auto run_tiles = [&](int count, bool partial) {
+            const int ntiles = partial ? (count + 15) / 16 : count / 16;
+            for (int tt = 0; tt < ntiles; ++tt) {
+                const int pidx = tt * 16 + (lane & 15);
+                const bool valid = pidx < count;
+                const uint32_t j = valid ? hits[pidx] : s;
+                bf16x8 bx;
+#pragma unroll
+                for (int e = 0; e < 8; ++e) bx[e] = (__bf16)0.f;
+                if (lane < 16) {
+                    const float4 xj = P[j], vj = V[j];
+                    const float rj = rho[j];
+                    bx[0] = (__bf16)((xi.x - xj.x) * inv_h);
+                    bx[1] = (__bf16)((xi.y - xj.y) * inv_h);
+                    bx[2] = (__bf16)((xi.z - xj.z) * inv_h);
+                    bx[3] = (__bf16)((vi.x - vj.x) * nm.v_scale);
+                    bx[4] = (__bf16)((vi.y - vj.y) * nm.v_scale);
+                    bx[5] = (__bf16)((vi.z - vj.z) * nm.v_scale);
+                    bx[6] = (__bf16)(rhoi * nm.rho_scale);
+                    bx[7] = (__bf16)(rj * nm.rho_scale);
+                } else if (lane < 32) {
+                    const float2 dj = D2[j];
+                    bx[0] = (__bf16)((pi + dj.x) * (0.5f * dj.y) * nm.pi_scale);
+                }
+                f32x4 h1[4];
+#pragma unroll
+                for (int m = 0; m < 4; ++m) h1[m] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1[m], bx, c1[m], 0, 0, 0);
+                bf16x8 b2[2];
+#pragma unroll
+                for (int ks = 0; ks < 2; ++ks)
+#pragma unroll
+                    for (int mm = 0; mm < 2; ++mm)
+#pragma unroll
+                        for (int e = 0; e < 4; ++e) b2[ks][4 * mm + e] = (__bf16)gelu_poly(h1[2 * ks + mm][e]);
+                f32x4 h2[4];
+#pragma unroll
+                for (int m = 0; m < 4; ++m) {
+                    h2[m] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2[m][0], b2[0], c2[m], 0, 0, 0);
+                    h2[m] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2[m][1], b2[1], h2[m], 0, 0, 0);
+                }
+                bf16x8 b3[2];
+#pragma unroll
+                for (int ks = 0; ks < 2; ++ks)
+#pragma unroll
+                    for (int mm = 0; mm < 2; ++mm)
+#pragma unroll
+                        for (int e = 0; e < 4; ++e) b3[ks][4 * mm + e] = (__bf16)gelu_poly(h2[2 * ks + mm][e]);
+                f32x4 o = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a3[0], b3[0], c3, 0, 0, 0);
+                o = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a3[1], b3[1], o, 0, 0, 0);
+                if (lane < 16 && valid) {
+                    acc0 += o[0];
+                    acc1 += o[1];
+                    acc2 += o[2];
+                }
+            }
+            // keep the unprocessed tail at the front of the list
+            const int done = ntiles * 16;
+            const int rest = count - done;
+            if (!partial && rest > 0) {
+                const uint32_t v = lane < rest ? hits[done + lane] : 0u;
+                if (lane < rest) hits[lane] = v;
+            }
+            return partial ? 0 : rest;
+        };
+
+        int cx, cy, cz;
+        float fx, fy, fz;
+        if (cell_coords(sc, xi.x, xi.y, xi.z, fx, fy, fz, cx, cy, cz)) {
+            const int dx = sc.dims[0], dy = sc.dims[1], dz = sc.dims[2];
+            const int xa = cx > SUB ? cx - SUB : 0, xb = cx < dx - SUB ? cx + SUB : dx - 1;
+            const int z0 = cz > SUB ? cz - SUB : 0, z1 = cz < dz - SUB ? cz + SUB : dz - 1;
+            const int y0 = cy > SUB ? cy - SUB : 0, y1 = cy < dy - SUB ? cy + SUB : dy - 1;
+            constexpr float kReach2 = (float)(SUB * SUB);
+            for (int z = z0; z <= z1; ++z) {
+                const float gz = z > cz ? (float)z - fz : (z < cz ? fz - (float)(z + 1) : 0.f);
+                for (int y = y0; y <= y1; ++y) {
+                    const float gy = y > cy ? (float)y - fy : (y < cy ? fy - (float)(y + 1) : 0.f);
+                    const float rem = kReach2 - fmaf(gy, gy, gz * gz);
+                    if (!(rem > 0.f)) continue;
+                    const float w = __builtin_amdgcn_sqrtf(rem);
+                    const int x0 = (int)fmaxf(fminf(floorf(fx - w), (float)(dx - 1)), (float)xa);
+                    const int x1 = (int)fminf(fmaxf(floorf(fx + w), 0.f), (float)xb);
+                    const uint32_t row = sc.cell_base + ((uint32_t)z * (uint32_t)dy + (uint32_t)y) * (uint32_t)dx;
+                    const uint32_t b = start[row + (uint32_t)x0], e = start[row + (uint32_t)(x1 + 1)];
+                    for (uint32_t base = b; base < e; base += 64u) {
+                        const uint32_t j = base + (uint32_t)lane;
+                        bool hit = false;
+                        if (j < e && j != s) {
+                            const float4 xj = P[j];
+                            const float ddx = xi.x - xj.x, ddy = xi.y - xj.y, ddz = xi.z - xj.z;
+                            hit = fmaf(ddx, ddx, fmaf(ddy, ddy, ddz * ddz)) < h2;
+                        }
+                        const uint64_t mask = __ballot(hit);
+                        const int posn = nh + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
+                                                                            __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+                        if (hit) hits[posn] = j;
+                        nh += __popcll(mask);
+                        if (nh > kHitCap - 64) nh = run_tiles(nh, false);
+                    }
+                }
+            }
+        }
+        run_tiles(nh, true);
+        // sum the 16 pair lanes (fixed tree order)
+#pragma unroll
+        for (int d = 8; d > 0; d >>= 1) {
+            acc0 += __shfl_xor(acc0, d);
+            acc1 += __shfl_xor(acc1, d);
+            acc2 += __shfl_xor(acc2, d);
+        }
+        if (lane == 0) {
+            pred[s * 3] = acc0;
+            pred[s * 3 + 1] = acc1;
+            pred[s * 3 + 2] = acc2;
+        }
+    }
+}
+
 // pred of slot s scattered to AoS order (diagnostics).
 __global__ __launch_bounds__(256) void k_pred_to_aos(const float* __restrict__ pred, const uint32_t* __restrict__ perm,
                                                      uint32_t n, float* __restrict__ out)
@@ -572,10 +806,29 @@ hipError_t launch_coarse_integrate(hipStream_t st, const DeviceState& d, const M
     return hipGetLastError();
 }
 
-hipError_t launch_coarse_mlp(hipStream_t st, const DeviceState& d, const MlpWork& w, int path)
+hipError_t launch_mlp_pack(hipStream_t st, const MlpWork& w)
 {
-    (void)path;
+    hipLaunchKernelGGL(k_mlp_pack, dim3(1), dim3(256), 0, st, w.params, w.pack_a, w.pack_c);
+    return hipGetLastError();
+}
+
+hipError_t launch_coarse_mlp(hipStream_t st, const DeviceState& d, int nscenes, const MlpWork& w, int path)
+{
     const int sub = d.tuning.cells_per_h;
+    if (path == 1) {
+        const dim3 grid(std::min<uint32_t>((d.n + 3u) / 4u, 256u * 64u)), block(256);
+#define NPRSPH_MFMA(S)                                                                                             \
+    hipLaunchKernelGGL(k_coarse_mlp_mfma<S>, grid, block, 0, st, d.scenes, nscenes, d.start, d.PA, d.VA, d.rho, d.D2, \
+                       w.pack_a, w.pack_c, w.norm, d.n, w.pred)
+        if (sub == 1)
+            NPRSPH_MFMA(1);
+        else if (sub == 2)
+            NPRSPH_MFMA(2);
+        else
+            NPRSPH_MFMA(3);
+#undef NPRSPH_MFMA
+        return hipGetLastError();
+    }
     if (sub == 1)
         hipLaunchKernelGGL(k_coarse_mlp_fp32<1>, dim3(d.ntiles), dim3(256), 0, st, d.tiles, d.scenes, d.start, d.PA,
                            d.VA, d.rho, d.D2, w.params, w.norm, w.pred);

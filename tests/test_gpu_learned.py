@@ -160,7 +160,7 @@ def test_coarse_step_matches_fp64_integrator(pkg, orc, ref, gpu_lib, use_network
     sim.mlp_set(params, _norm(pkg))
     s0 = sim.download()
     if use_network:
-        pred = sim.mlp_predict(path=0).astype(np.float64)
+        pred = sim.mlp_predict(path=1).astype(np.float64)  # the coarse step runs the MFMA kernel
         acc = NORM["acc_scale"] * pred + np.array([0.0, np.float32(-9806.65), 0.0])
     else:
         sim.rho_pres()
@@ -203,3 +203,27 @@ def test_judge_zero_limit_keeps_plain_result(pkg, orc, ref, gpu_lib):
     st = sim.coarse_step(4, use_network=True, judge_every=4, drift_limit=0.0, cooldown=3)
     assert st.fallbacks == 1 and st.plain_steps == 16 and st.coarse_steps == 0
     sim.close()
+
+
+def test_predict_mfma_matches_fp32(pkg, orc, ref, gpu_lib):
+    """BF16 MFMA messages equal the FP32 path to BF16 accuracy, on a scene with
+    more than 512 neighbour hits for some particles (hit-list wrap-around)."""
+    sim, sc = _scene(pkg, orc, n=4000, steps=5)
+    params = ref.init_params(19)
+    params[ref.NPARAMS - 195:] += np.random.default_rng(20).normal(0, 0.05, 195).astype(np.float32)
+    sim.mlp_set(params, _norm(pkg))
+    p32 = sim.mlp_predict(path=0)
+    pbf = sim.mlp_predict(path=1)
+    assert np.all(np.isfinite(pbf))
+    rel = np.linalg.norm(pbf - p32) / np.linalg.norm(p32)
+    assert rel < 2e-2, rel
+    # crowded: every particle within h of every other (hit lists longer than one LDS list)
+    n = 700
+    with pkg.Simulation(n) as dense:
+        p = orc.particles(n)
+        p["pos"][:, :3] = np.random.default_rng(1).uniform(0.0, 0.008, (n, 3)).astype(np.float32)
+        dense.upload(p)
+        dense.mlp_set(params, _norm(pkg))
+        a = dense.mlp_predict(path=0)
+        b = dense.mlp_predict(path=1)
+        assert np.linalg.norm(b - a) / np.linalg.norm(a) < 2e-2

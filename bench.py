@@ -77,17 +77,24 @@ def cpu_baseline(pkg, scene, budget_s: float):
                       f"step, {threads} OpenMP threads ({el:.1f} s)"}
 
 
+def progress(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def density_timeline(sim, total_fine, every_fine, coarse=None):
     """Mean density (scene 0) every `every_fine` fine steps of simulated time,
-    advancing with plain steps or with coarse steps (dict of coarse options)."""
+    advancing with plain steps or with coarse steps (dict of coarse options).
+    Also returns the final kinetic energy and centre of mass."""
     out = []
+    m = None
     for _ in range(total_fine // every_fine):
         if coarse is None:
             sim.step(every_fine)
         else:
             sim.coarse_step(every_fine // 4, **coarse)
-        out.append(sim.metrics(0).mean_rho)
-    return np.array(out)
+        m = sim.metrics(0)
+        out.append(m.mean_rho)
+    return np.array(out), m.kinetic, np.array(m.com)
 
 
 def learned_form(pkg, scenes, warmup, fine_steps, plain_ms_per_step, weights_path=None):
@@ -115,15 +122,16 @@ def learned_form(pkg, scenes, warmup, fine_steps, plain_ms_per_step, weights_pat
     el = time.perf_counter() - t0
     n = sum(sc.count for sc in scenes)
     ms_per_coarse = 1e3 * el / ncoarse
-    # accuracy: mean density every 20 fine steps of simulated time
-    every = 20
+    progress(f"learned: {ncoarse} coarse steps in {el:.2f} s; accuracy timelines")
+    # accuracy: mean density every `every` fine steps of simulated time
+    every = 20 if n <= 2_000_000 else 100
     sim.upload(s0)
-    plain = density_timeline(sim, fine_steps, every)
+    plain, ke_p, com_p = density_timeline(sim, fine_steps, every)
     sim.upload(s0)
-    learned = density_timeline(sim, fine_steps, every, coarse=judge)
+    learned, ke_l, com_l = density_timeline(sim, fine_steps, every, coarse=judge)
     sim.upload(s0)
-    ablation = density_timeline(sim, fine_steps, every,
-                                coarse=dict(use_network=False, judge_every=0, drift_limit=0.0, cooldown=0))
+    ablation, ke_a, com_a = density_timeline(sim, fine_steps, every,
+                                             coarse=dict(use_network=False, judge_every=0, drift_limit=0.0, cooldown=0))
     sim.close()
 
     def err(x):
@@ -131,6 +139,9 @@ def learned_form(pkg, scenes, warmup, fine_steps, plain_ms_per_step, weights_pat
         if not ok.all():
             return None
         return float(np.mean(np.abs(x - plain) / plain))
+
+    def fin(x):
+        return float(x) if np.isfinite(x) else None
 
     plain_ms_per_4 = 4.0 * plain_ms_per_step
     return {
@@ -146,6 +157,10 @@ def learned_form(pkg, scenes, warmup, fine_steps, plain_ms_per_step, weights_pat
                             "same state; above 0.01*rho0 the plain result is kept",
         "mean_density_rel_error_vs_plain": err(learned),
         "no_network_mean_density_rel_error_vs_plain": err(ablation),
+        "final_kinetic_rel_error_vs_plain": fin(abs(ke_l - ke_p) / ke_p),
+        "final_com_error_vs_plain": fin(np.linalg.norm(com_l - com_p)),
+        "no_network_final_kinetic_rel_error_vs_plain": fin(abs(ke_a - ke_p) / ke_p),
+        "no_network_final_com_error_vs_plain": fin(np.linalg.norm(com_a - com_p)),
         "timeline_every_fine_steps": every,
         "weights": {"rollouts": meta.get("rollouts"), "updates": meta.get("updates"), "loss_last20": meta.get("loss_last20")},
     }
@@ -205,6 +220,7 @@ def main(argv=None):
     pkg = entry.load_package()
     scenes = _scene(pkg, args.workload)
     n = sum(s.count for s in scenes)
+    progress(f"{args.workload}: {n} particles in {len(scenes)} scene(s)")
     sim = make_sim(pkg, scenes)
     if args.tune not in ("auto", "default"):
         sim.set_tuning(*[int(v) for v in args.tune.split(",")])
@@ -215,6 +231,7 @@ def main(argv=None):
     tuning = sim.tuning().as_dict()
     sim.step(1)  # capture the step graphs of the chosen tuning outside the timed region
     sim.synchronize()
+    progress(f"tuning {tuning}; timing {args.steps} steps")
     t0 = time.perf_counter()
     sim.step(args.steps)
     sim.synchronize()
@@ -241,6 +258,7 @@ def main(argv=None):
     m = sim.metrics(0)
     sim.close()
 
+    progress(f"plain: {ms_per_step:.3f} ms/step")
     learned = None
     if not args.no_learned and len(scenes) == 1:
         learned = learned_form(pkg, scenes, args.warmup, args.steps, ms_per_step)

@@ -120,9 +120,11 @@ typedef struct sph_tuning {
     int32_t blocks_per_xcd;  /* 0: one workgroup per tile; >0: persistent workgroups per XCD */
 } sph_tuning_t;
 
-/* Per-scene reductions over the current state (sph_metrics). */
+/* Per-scene reductions over the current state (sph_metrics).  Kinetic
+ * energy, centre of mass and max speed are taken over the particles whose
+ * pos and vel are finite. */
 typedef struct sph_metrics {
-    double mean_rho;      /* mean of extras.x */
+    double mean_rho;      /* mean of extras.x over all particles */
     double kinetic;       /* sum of 0.5*mass*|vel|^2 */
     double com[3];        /* mean position */
     double max_speed;     /* max |vel| */

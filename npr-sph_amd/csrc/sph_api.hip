@@ -902,11 +902,12 @@ int sph_metrics(sph_ctx* c, int scene, sph_metrics_t* out)
     for (int t = 0; t < nt; ++t)
         for (int k = 0; k < 7; ++k) acc[k] = (k == 5) ? std::max(acc[k], part[(size_t)t * 8 + k]) : acc[k] + part[(size_t)t * 8 + k];
     const double cnt = (double)c->count[scene];
+    const double fin = std::max(cnt - acc[6], 1.0);
     out->mean_rho = acc[0] / cnt;
     out->kinetic = acc[1];
-    out->com[0] = acc[2] / cnt;
-    out->com[1] = acc[3] / cnt;
-    out->com[2] = acc[4] / cnt;
+    out->com[0] = acc[2] / fin;
+    out->com[1] = acc[3] / fin;
+    out->com[2] = acc[4] / fin;
     out->max_speed = acc[5];
     out->nonfinite = (uint64_t)acc[6];
     return SPH_OK;

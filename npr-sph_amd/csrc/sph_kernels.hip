@@ -512,7 +512,8 @@ __global__ __launch_bounds__(kTile) void k_integrate_aos(const Tile* __restrict_
 // ------------------------------------------------------------- reductions ---
 
 // Per-tile partial sums over the AoS: rho, kinetic energy, position, max
-// speed, count of non-finite pos/vel.  Eight doubles per tile.
+// speed, count of non-finite pos/vel (kinetic energy, position and speed over
+// finite particles only).  Eight doubles per tile.
 __global__ __launch_bounds__(kTile) void k_metrics(const Tile* __restrict__ tiles, const SceneDev* __restrict__ scenes,
                                                    const sph_particle_t* __restrict__ aos, double* __restrict__ partials,
                                                    int first_tile)
@@ -529,10 +530,10 @@ __global__ __launch_bounds__(kTile) void k_metrics(const Tile* __restrict__ tile
         const bool finite = isfinite(p.x) && isfinite(p.y) && isfinite(p.z) && isfinite(vel.x) && isfinite(vel.y) &&
                             isfinite(vel.z);
         v[0] = e.x;
-        v[1] = 0.5 * (double)sc.mass * s2;
-        v[2] = p.x;
-        v[3] = p.y;
-        v[4] = p.z;
+        v[1] = finite ? 0.5 * (double)sc.mass * s2 : 0.0;
+        v[2] = finite ? p.x : 0.0;
+        v[3] = finite ? p.y : 0.0;
+        v[4] = finite ? p.z : 0.0;
         v[5] = finite ? sqrt(s2) : 0.0;
         v[6] = finite ? 0.0 : 1.0;
     }

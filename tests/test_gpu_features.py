@@ -208,3 +208,14 @@ def test_metrics_match_numpy(pkg, orc, gpu_lib):
     assert m.max_speed == pytest.approx(np.sqrt((v * v).sum(1)).max(), rel=1e-12)
     assert np.allclose(m.com, st["pos"][:, :3].astype(np.float64).mean(0), rtol=1e-10)
     assert m.nonfinite == 0
+    # a NaN particle is counted and left out of the kinetic energy and centre of mass
+    st2 = st.copy()
+    st2["vel"][5, 0] = np.nan
+    with pkg.Simulation(sc.count, boundaries=pkg.sph.make_boundary(sc.lower, sc.upper)) as sim:
+        sim.upload(st2)
+        m2 = sim.metrics(0)
+    keep = np.ones(len(st), bool)
+    keep[5] = False
+    assert m2.nonfinite == 1 and np.isfinite(m2.kinetic)
+    assert m2.kinetic == pytest.approx(0.5 * float(np.float32(0.02)) * (v[keep] ** 2).sum(), rel=1e-10)
+    assert np.allclose(m2.com, st["pos"][keep, :3].astype(np.float64).mean(0), rtol=1e-10)

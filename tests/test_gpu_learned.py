@@ -227,3 +227,45 @@ def test_predict_mfma_matches_fp32(pkg, orc, ref, gpu_lib):
         a = dense.mlp_predict(path=0)
         b = dense.mlp_predict(path=1)
         assert np.linalg.norm(b - a) / np.linalg.norm(a) < 2e-2
+
+
+def test_trained_network_beats_no_network(pkg, orc, gpu_lib):
+    """The committed weights (trained on the GPU on 200 dam1m rollouts) predict
+    the 4-step mean acceleration of held-out states better than the method
+    without the network (the SPH force held over the 4 steps) and than zero."""
+    import sys
+    sys.path.insert(0, os.path.join(ROOT, "npr-sph_amd"))
+    import train_surrogate
+    params, norm, meta = train_surrogate.load_weights()
+    assert meta["rollouts"] >= 200 and meta["workload"] == "dam1m"
+    sph = pkg.sph
+    sc = pkg.scenes.dam1m(seed=4242)  # seed not used in training
+    b = sph.make_boundary(sc.lower, sc.upper)
+    G = np.array([0.0, np.float32(-9806.65), 0.0])
+    dt = float(np.float32(1.0) / np.float32(10000.0))
+    with pkg.Simulation(sc.count, boundaries=b) as sim, pkg.Simulation(sc.count, boundaries=b) as later:
+        sim.init_lattice(sph.make_lattice(**sc.lattice_args()))
+        sim.step(600)
+        later.copy_state_from(sim)
+        later.step(4)
+        sim.mlp_set(params, norm)
+        pred = sim.mlp_predict(path=1).astype(np.float64)
+        a, l = sim.download(), later.download()
+        sim.rho_pres()
+        sim.force()
+        f = sim.download()
+    with np.errstate(invalid="ignore", divide="ignore"):
+        sph_acc = f["force"][:, :3].astype(np.float64) / f["extras"][:, [0]] - G
+    tgt = (l["vel"][:, :3].astype(np.float64) - a["vel"][:, :3]) / (4 * dt) - G
+    lo, hi = np.array(sc.lower, np.float32), np.array(sc.upper, np.float32)
+    ok = np.all(np.isfinite(tgt), 1) & np.all(np.isfinite(sph_acc), 1)
+    for st in (a, l):
+        ps = st["pos"][:, :3]
+        ok &= np.all((np.abs(ps - lo) > 1e-6) & (np.abs(ps - hi) > 1e-6), 1)
+    A = norm.acc_scale
+
+    def loss(p):
+        return float(np.mean(np.sum((p[ok] - tgt[ok] / A) ** 2, 1)))
+
+    l_net, l_sph, l_zero = loss(pred), loss(sph_acc / A), loss(np.zeros_like(tgt))
+    assert l_net < 0.5 * l_zero and l_net < 0.1 * l_sph, (l_net, l_sph, l_zero)

@@ -197,3 +197,48 @@ def test_dam1m_one_step_vs_grid_oracle(pkg, orc, gpu_lib):
         g = orc.grid_geometry(orc.smoothing_length(), ob, sc.count, sim.grid_info(0).cells_per_h)
         _, cstart, cperm = orc.grid_build(s2["pos"][:, :3], g)
         assert_same_grid(start, perm, cstart, cperm, [g.ncells])
+
+
+def test_reference_make_grid_scene(pkg, orc, gpu_lib):
+    """The reference's literal initial state (make_grid, Main.cpp:488-505: 10x100x10
+    at spacing PARTICLE_RADIUS = h/4, ~251 neighbours) through all three passes,
+    then 50 steps later, against the O(N^2) FP64 transcription."""
+    p0 = orc.reference_grid()
+    ob = orc.default_boundary()
+    with pkg.Simulation(10000) as sim:
+        sim.upload(p0)
+        for k in range(2):
+            st = sim.download()
+            sim.rho_pres()
+            s1 = sim.download()
+            check_density(orc, st, s1, ob, grid=False)
+            sim.force()
+            s2 = sim.download()
+            check_force(orc, s1, s2, ob, grid=False)
+            sim.integrate()
+            s3 = sim.download()
+            check_integrate(orc, s2, s3, ob)
+            if k == 0:
+                sim.step(50)
+
+
+def test_short_trajectory_statistics_vs_fp64(pkg, orc, gpu_lib):
+    """dam10k for 40 steps in FP32 on the GPU and in FP64 on the CPU from the same
+    state: the chaotic trajectories stay close in aggregate (mean density,
+    kinetic energy, centre of mass)."""
+    sc = pkg.scenes.dam10k()
+    p0 = _lattice(orc, sc)
+    ob = orc.make_boundary(sc.lower, sc.upper)
+    st = orc.State64(p0)
+    with pkg.Simulation(sc.count, boundaries=pkg.sph.make_boundary(sc.lower, sc.upper)) as sim:
+        sim.upload(p0)
+        for _ in range(4):
+            sim.step(10)
+            st.step(b=ob, grid=True, nsteps=10)
+            g = sim.download()
+            gv = g["vel"][:, :3].astype(np.float64)
+            cv = st.v.reshape(-1, 3)
+            ke_g, ke_c = (gv * gv).sum(), (cv * cv).sum()
+            assert abs(ke_g - ke_c) <= 1e-3 * ke_c
+            assert abs(g["extras"][:, 0].astype(np.float64).mean() - st.rho.mean()) <= 1e-4 * st.rho.mean()
+            assert np.allclose(g["pos"][:, :3].astype(np.float64).mean(0), st.x.reshape(-1, 3).mean(0), atol=1e-5)

@@ -604,7 +604,9 @@ int sph_create_batch(sph_ctx** out, int nscenes, const size_t* counts, const sph
     DeviceState& d = c->d;
     d.n = n;
     d.ntiles = (int)c->tiles_h.size();
-    bool ok = dalloc(&d.aos, n) == hipSuccess && dalloc(&d.PA, n) == hipSuccess && dalloc(&d.VA, n) == hipSuccess &&
+    // PA and VA carry one extra slot, the force pass's inert sentinel particle.
+    bool ok = dalloc(&d.aos, n) == hipSuccess && dalloc(&d.PA, (size_t)n + 1u) == hipSuccess &&
+              dalloc(&d.VA, (size_t)n + 1u) == hipSuccess &&
               dalloc(&d.PB, n) == hipSuccess && dalloc(&d.VB, n) == hipSuccess && dalloc(&d.rho, n) == hipSuccess &&
               dalloc(&d.D2, n) == hipSuccess && dalloc(&d.perm, n) == hipSuccess && dalloc(&d.key, n) == hipSuccess &&
               dalloc(&d.slot, n) == hipSuccess && dalloc(&d.torig, n) == hipSuccess &&
@@ -631,7 +633,12 @@ int sph_create_batch(sph_ctx** out, int nscenes, const size_t* counts, const sph
     }
     int rc = apply_params(c, true);
     if (rc != SPH_OK) return bail(rc);
-    if (launch_fill_aos_defaults(c->stream, d.aos, n) != hipSuccess || launch_identity(c->stream, d.perm, n) != hipSuccess ||
+    // Sentinel: far from every particle (finite distances), p = 0, 1/rho = 0.
+    static const float4 kSentinelP = make_float4(1e10f, 1e10f, 1e10f, 0.f);
+    static const float4 kSentinelV = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (hipMemcpyAsync(d.PA + n, &kSentinelP, sizeof(float4), hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+        hipMemcpyAsync(d.VA + n, &kSentinelV, sizeof(float4), hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+        launch_fill_aos_defaults(c->stream, d.aos, n) != hipSuccess || launch_identity(c->stream, d.perm, n) != hipSuccess ||
         hipStreamSynchronize(c->stream) != hipSuccess) {
         (void)hipGetLastError();
         c->err = "initialisation kernels failed";

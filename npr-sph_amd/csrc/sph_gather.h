@@ -61,4 +61,123 @@ __device__ __forceinline__ void for_each_row(const SceneDev& sc, const uint32_t*
     }
 }
 
+// ------------------------------------------------------ LDS-staged walk ---
+
+// Candidates staged per wave per round.
+constexpr int kStage = 128;
+
+__device__ __forceinline__ int wave_min_i32(int v)
+{
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) v = min(v, __shfl_xor(v, d));
+    return v;
+}
+
+__device__ __forceinline__ int wave_max_i32(int v)
+{
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) v = max(v, __shfl_xor(v, d));
+    return v;
+}
+
+// Orders this wave's LDS accesses on either side: a wave's LDS instructions
+// complete in issue order, and the fences keep the compiler from moving
+// accesses across this point.
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Per-wave LDS words a gather needs: the row table of for_each_row (stride
+// 64) or the candidate stage, never both at once.
+template <int SUB>
+constexpr int gather_lds_words()
+{
+    return 2 * (2 * SUB + 1) * 64 > 4 * kStage ? 2 * (2 * SUB + 1) * 64 : 4 * kStage;
+}
+
+// Stencil walk of one wave.  When every valid lane of the wave lies in the
+// same cell row (cy, cz) - the common case, since slots are cell-sorted x
+// fastest - the slot range of each stencil row over the wave's x cells
+// widened by SUB is contiguous.  That range is copied into the wave's LDS
+// stage kStage candidates per round, lane k loading slots k, k+64, ...
+// (coalesced), and each lane then reads its own pruned sub-range of the round
+// from LDS.  Otherwise every lane walks its own rows through global memory
+// (for_each_row).  Both ways visit the same candidates in the same order
+// (rows in z, y order, slots ascending), so results do not depend on the way
+// taken.  Lanes with valid = false visit nothing but must take part: the call
+// is made by the whole wave.
+// body(q, m, j) visits the m candidates q[0..m) whose first slot is j; q
+// points into LDS or into P.
+template <int SUB, bool PRUNE, class Body>
+__device__ __forceinline__ void gather_rows(const SceneDev& sc, const uint32_t* __restrict__ start,
+                                            const float4* __restrict__ P, uint32_t* lds, bool valid, float fx,
+                                            float fy, float fz, int cx, int cy, int cz, Body&& body)
+{
+    constexpr int kRows = 2 * SUB + 1;
+    const int lane = (int)__lane_id();
+    const int dx = sc.dims[0], dy = sc.dims[1], dz = sc.dims[2];
+    const int rk = cz * dy + cy;
+    const int rmin = wave_min_i32(valid ? rk : 0x7fffffff);
+    const int rmax = wave_max_i32(valid ? rk : -1);
+    if (rmin != rmax) {
+        if (valid)
+            for_each_row<SUB, PRUNE, 64>(sc, start, fx, fy, fz, cx, cy, cz, lds + lane,
+                                         [&](uint32_t b, uint32_t e) { body(P + b, e > b ? e - b : 0u, b); });
+        return;
+    }
+    if (rmin == 0x7fffffff) return;  // no valid lane
+    // Wave-uniform row and x extent.
+    const int row_yz = __builtin_amdgcn_readfirstlane(rmin);
+    const int ucy = row_yz % dy, ucz = row_yz / dy;
+    const int xlo = __builtin_amdgcn_readfirstlane(max(wave_min_i32(valid ? cx : 0x7fffffff) - SUB, 0));
+    const int xhi = __builtin_amdgcn_readfirstlane(min(wave_max_i32(valid ? cx : -1) + SUB, dx - 1));
+    float4* stage = reinterpret_cast<float4*>(lds);
+    const int xa = cx > SUB ? cx - SUB : 0;
+    const int xb = cx < dx - SUB ? cx + SUB : dx - 1;
+    const int z0 = ucz > SUB ? ucz - SUB : 0, z1 = ucz < dz - SUB ? ucz + SUB : dz - 1;
+    constexpr float kReach2 = (float)(SUB * SUB);
+    for (int z = z0; z <= z1; ++z) {
+        const float gz = z > cz ? (float)z - fz : (z < cz ? fz - (float)(z + 1) : 0.f);
+        uint32_t rb[kRows], re[kRows], RB[kRows], RE[kRows];
+#pragma unroll
+        for (int k = 0; k < kRows; ++k) {
+            const int y = ucy - SUB + k;
+            rb[k] = re[k] = RB[k] = RE[k] = 0;
+            if (y >= 0 && y < dy) {
+                const uint32_t row = sc.cell_base + ((uint32_t)z * (uint32_t)dy + (uint32_t)y) * (uint32_t)dx;
+                RB[k] = start[row + (uint32_t)xlo];
+                RE[k] = start[row + (uint32_t)xhi + 1u];
+                int x0 = xa, x1 = xb;
+                bool keep = valid;
+                if constexpr (PRUNE) {
+                    const float gy = y > cy ? (float)y - fy : (y < cy ? fy - (float)(y + 1) : 0.f);
+                    const float rem = kReach2 - fmaf(gy, gy, gz * gz);
+                    keep = keep && rem > 0.f;
+                    const float w = __builtin_amdgcn_sqrtf(fmaxf(rem, 0.f));
+                    x0 = (int)fmaxf(fminf(floorf(fx - w), (float)(dx - 1)), (float)xa);
+                    x1 = (int)fminf(fmaxf(floorf(fx + w), 0.f), (float)xb);
+                }
+                if (keep) {
+                    rb[k] = start[row + (uint32_t)x0];
+                    re[k] = start[row + (uint32_t)(x1 + 1)];
+                }
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < kRows; ++k) {
+            for (uint32_t c0 = RB[k]; c0 < RE[k]; c0 += kStage) {
+                const uint32_t n = min(RE[k] - c0, (uint32_t)kStage);
+                for (uint32_t r = (uint32_t)lane; r < n; r += 64u) stage[r] = P[c0 + r];
+                wave_lds_sync();
+                const uint32_t lo = max(rb[k], c0), hi = min(re[k], c0 + n);
+                if (hi > lo) body(stage + (lo - c0), hi - lo, lo);
+                wave_lds_sync();
+            }
+        }
+    }
+}
+
 }  // namespace nprsph

@@ -147,10 +147,12 @@ __global__ __launch_bounds__(256) void k_lattice(sph_particle_t* __restrict__ ao
 }
 
 // Copy AoS state into slot order B (pos/vel) and/or A (rho, p) through perm.
+// The extras also go to the w lanes of A: PA.w = p, VA.w = 1/rho.
 __global__ __launch_bounds__(kTile) void k_refresh(const Tile* __restrict__ tiles, const sph_particle_t* __restrict__ aos,
                                                    const uint32_t* __restrict__ perm, float4* __restrict__ P,
                                                    float4* __restrict__ V, float* __restrict__ rho,
-                                                   float2* __restrict__ D2, int pos_vel, int extras)
+                                                   float2* __restrict__ D2, float* __restrict__ PAw,
+                                                   float* __restrict__ VAw, int pos_vel, int extras)
 {
     const Tile t = tiles[blockIdx.x];
     const uint32_t s = t.begin + threadIdx.x;
@@ -163,8 +165,11 @@ __global__ __launch_bounds__(kTile) void k_refresh(const Tile* __restrict__ tile
     }
     if (extras) {
         const float4 e = r[3];
+        const float inv = 1.0f / e.x;
         rho[s] = e.x;
-        D2[s] = make_float2(e.y, 1.0f / e.x);
+        D2[s] = make_float2(e.y, inv);
+        PAw[4 * (size_t)s + 3] = e.y;
+        VAw[4 * (size_t)s + 3] = inv;
     }
 }
 
@@ -323,49 +328,56 @@ __device__ __forceinline__ void for_each_gather_tile(uint32_t ntiles, F&& f)
 
 // rho_pres_comp.glsl:35-58.  rho_i = m*315/(64 pi h^9) * sum_{r<h} (h^2-r^2)^3
 // over the stencil (self included, :46); p_i = max(k(rho_i-rho0), 0).
+// Besides rho and (p, 1/rho) the pass stores p in P[s].w and 1/rho in
+// V[s].w, so that the force pass gets a neighbour's whole record in two
+// 16-byte loads.  Candidates are read through P's xyz lanes only; the w lanes
+// are written through Pw/Vw (the same buffers, as floats), so no lane is both
+// read and written by the pass.
 template <int SUB, bool PRUNE, int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_density(const Tile* __restrict__ tiles, const SceneDev* __restrict__ scenes,
                                                    const uint32_t* __restrict__ start, const float4* __restrict__ P,
-                                                   float* __restrict__ rho_out, float2* __restrict__ D2,
-                                                   const uint32_t* __restrict__ perm, sph_particle_t* __restrict__ aos,
-                                                   uint32_t ntiles)
+                                                   float* __restrict__ rho_out, float2* __restrict__ D2, float* Pw,
+                                                   float* Vw, const uint32_t* __restrict__ perm,
+                                                   sph_particle_t* __restrict__ aos, uint32_t ntiles)
 {
-    __shared__ uint32_t rows[2 * (2 * SUB + 1) * BLOCK];
+    __shared__ uint32_t lds[BLOCK / 64][gather_lds_words<SUB>()];
     for_each_gather_tile(ntiles, [&](uint32_t ti) {
         const Tile t = tiles[ti];
         const SceneDev& sc = scenes[t.scene];
         const uint32_t s = t.begin + threadIdx.x;
-        if (s >= t.end) return;
-        const float4 xi = P[s];
+        const bool active = s < t.end;
+        const float4 xi = P[active ? s : t.begin];
         const float h2 = sc.h2;
         float acc = 0.f;
-        int cx, cy, cz;
-        float fx, fy, fz;
-        // Term (h^2-r^2)^3 of one candidate; r >= h and NaN/Inf distances add nothing.
+        int cx = 0, cy = 0, cz = 0;
+        float fx = 0.f, fy = 0.f, fz = 0.f;
+        // Term (h^2-r^2)^3 of one candidate, h^2-r^2 formed as h^2-dz^2-dy^2-dx^2
+        // by fused steps; r >= h and NaN/Inf distances add nothing.
         auto term = [&](const float4 xj) {
             const float ddx = xi.x - xj.x, ddy = xi.y - xj.y, ddz = xi.z - xj.z;
-            const float r2 = fmaf(ddx, ddx, fmaf(ddy, ddy, ddz * ddz));
-            const float w = fmaxf(h2 - r2, 0.f);
+            const float w = fmaxf(fmaf(-ddx, ddx, fmaf(-ddy, ddy, fmaf(-ddz, ddz, h2))), 0.f);
             return w * w * w;
         };
-        if (cell_coords(sc, xi.x, xi.y, xi.z, fx, fy, fz, cx, cy, cz)) {
-            for_each_row<SUB, PRUNE, BLOCK>(sc, start, fx, fy, fz, cx, cy, cz, rows + threadIdx.x,
-                                            [&](uint32_t b, uint32_t e) {
-                                                uint32_t j = b;
-                                                for (; j + 4 <= e; j += 4) {
-                                                    const float4 x0 = P[j], x1 = P[j + 1], x2 = P[j + 2], x3 = P[j + 3];
-                                                    acc += term(x0);
-                                                    acc += term(x1);
-                                                    acc += term(x2);
-                                                    acc += term(x3);
-                                                }
-                                                for (; j < e; ++j) acc += term(P[j]);
-                                            });
-        }
+        const bool valid = active && cell_coords(sc, xi.x, xi.y, xi.z, fx, fy, fz, cx, cy, cz);
+        gather_rows<SUB, PRUNE>(sc, start, P, lds[threadIdx.x / 64], valid, fx, fy, fz, cx, cy, cz,
+                                [&](const float4* q, uint32_t m, uint32_t) {
+                                    for (; m >= 4u; m -= 4u, q += 4) {
+                                        const float4 x0 = q[0], x1 = q[1], x2 = q[2], x3 = q[3];
+                                        acc += term(x0);
+                                        acc += term(x1);
+                                        acc += term(x2);
+                                        acc += term(x3);
+                                    }
+                                    for (; m > 0u; --m, ++q) acc += term(*q);
+                                });
+        if (!active) return;
         const float rho = acc * sc.poly6_m;
         const float p = fmaxf(sc.gas_const * (rho - sc.rho0), 0.f);
+        const float inv = 1.0f / rho;
         rho_out[s] = rho;
-        D2[s] = make_float2(p, 1.0f / rho);
+        D2[s] = make_float2(p, inv);
+        Pw[4 * (size_t)s + 3] = p;
+        Vw[4 * (size_t)s + 3] = inv;
         if (aos) {
             float* e = aos[perm[s]].extras;
             e[0] = rho;
@@ -376,58 +388,75 @@ __global__ __launch_bounds__(BLOCK) void k_density(const Tile* __restrict__ tile
 
 // Hits found while scanning candidates are queued per lane in LDS and then
 // processed by all lanes in lock step, so the pair-force arithmetic runs once
-// per neighbour instead of once per candidate of the slowest lane.
+// per neighbour instead of once per candidate of the slowest lane.  Every
+// candidate's index is written to the lane's next queue entry and the entry is
+// kept (count advanced) only for a hit, so the scan has no branches.
 constexpr int kQueue = 16;
+static_assert(kQueue % 2 == 0, "the queue is drained two entries at a time");
 
 // force_comp.glsl:35-66 over the stencil, j != i (:50-53), r < h (:57):
 //   pressure  += m(p_i+p_j)/(2 rho_j) * 45/(pi h^6) * (h-r)^2 * delta/r
 //   viscosity += m(v_j-v_i)/rho_j * 45/(pi h^6) * (h-r), then * visc
 //   F = pressure + viscosity + rho_i * G
+// Reads P = (x, y, z, p) and V = (vx, vy, vz, 1/rho) as left by the density
+// pass.  Slot `sentinel` (one past the last particle) holds a far-away
+// particle with p = 0 and 1/rho = 0: its pair terms are exact zeros, and it
+// fills the second entry of a queue pair that a lane does not have.
 // FUSE additionally runs integrate_comp.glsl:35-81 on the new force, writes the
 // AoS record (pos, vel, force, extras.xy) and the next step's B state.
 template <bool FUSE, int SUB, bool PRUNE, int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_force(const Tile* __restrict__ tiles, const SceneDev* __restrict__ scenes,
                                                  const uint32_t* __restrict__ start, const float4* __restrict__ P,
                                                  const float4* __restrict__ V, const float* __restrict__ rho,
-                                                 const float2* __restrict__ D2, const uint32_t* __restrict__ perm,
-                                                 sph_particle_t* __restrict__ aos, float4* __restrict__ PN,
-                                                 float4* __restrict__ VN, uint32_t ntiles)
+                                                 const uint32_t* __restrict__ perm, sph_particle_t* __restrict__ aos,
+                                                 float4* __restrict__ PN, float4* __restrict__ VN, uint32_t sentinel,
+                                                 uint32_t ntiles)
 {
     __shared__ uint32_t queue[kQueue][BLOCK];
-    __shared__ uint32_t rows[2 * (2 * SUB + 1) * BLOCK];
+    __shared__ uint32_t lds[BLOCK / 64][gather_lds_words<SUB>()];
     for_each_gather_tile(ntiles, [&](uint32_t ti) {
         const Tile t = tiles[ti];
         const SceneDev& sc = scenes[t.scene];
         const uint32_t s = t.begin + threadIdx.x;
-        if (s >= t.end) return;
-        const float4 xi = P[s];
-        const float4 vi = V[s];
-        const float pi = D2[s].x;
-        const float rhoi = rho[s];
+        const bool active = s < t.end;
+        const uint32_t si = active ? s : t.begin;
+        const float4 xi = P[si];
+        const float4 vi = V[si];
+        const float pi = xi.w;
+        const float rhoi = rho[si];
         const float h = sc.h, h2 = sc.h2;
         float ax = 0.f, ay = 0.f, az = 0.f;  // pressure sum
         float bx = 0.f, by = 0.f, bz = 0.f;  // viscosity sum
         int cnt = 0;
+        // One neighbour's pressure and viscosity terms.  The hardware reciprocal
+        // square root of r^2 = 0 (a coincident pair) is +inf, so r = 0*inf is
+        // NaN, as normalize(0) in the reference.
+        auto pair = [&](const float4 xj, const float4 vj) {
+            const float ddx = xi.x - xj.x, ddy = xi.y - xj.y, ddz = xi.z - xj.z;
+            const float r2 = fmaf(ddx, ddx, fmaf(ddy, ddy, ddz * ddz));
+            const float inv_r = __builtin_amdgcn_rsqf(r2);
+            const float r = r2 * inv_r;
+            const float q = h - r;
+            const float cp = (pi + xj.w) * vj.w * (q * q) * inv_r;
+            ax = fmaf(cp, ddx, ax);
+            ay = fmaf(cp, ddy, ay);
+            az = fmaf(cp, ddz, az);
+            const float cv = vj.w * q;
+            bx = fmaf(cv, vj.x - vi.x, bx);
+            by = fmaf(cv, vj.y - vi.y, by);
+            bz = fmaf(cv, vj.z - vi.z, bz);
+        };
+        // Drains the queue two entries per round; the loads of both entries
+        // are issued before either is used.
         auto flush = [&]() {
-            for (int k = 0; __any(k < cnt); ++k) {
+            for (int k = 0; __any(k < cnt); k += 2) {
                 if (k < cnt) {
-                    const uint32_t j = queue[k][threadIdx.x];
-                    const float4 xj = P[j];
-                    const float ddx = xi.x - xj.x, ddy = xi.y - xj.y, ddz = xi.z - xj.z;
-                    const float r2 = fmaf(ddx, ddx, fmaf(ddy, ddy, ddz * ddz));
-                    const float inv_r = rsqrtf(r2);  // r == 0 (coincident pair) gives NaN, as normalize(0)
-                    const float r = r2 * inv_r;
-                    const float q = h - r;
-                    const float2 dj = D2[j];
-                    const float4 vj = V[j];
-                    const float cp = (pi + dj.x) * dj.y * (q * q) * inv_r;
-                    ax = fmaf(cp, ddx, ax);
-                    ay = fmaf(cp, ddy, ay);
-                    az = fmaf(cp, ddz, az);
-                    const float cv = dj.y * q;
-                    bx = fmaf(cv, vj.x - vi.x, bx);
-                    by = fmaf(cv, vj.y - vi.y, by);
-                    bz = fmaf(cv, vj.z - vi.z, bz);
+                    const uint32_t j0 = queue[k][threadIdx.x];
+                    const uint32_t j1 = k + 1 < cnt ? queue[k + 1][threadIdx.x] : sentinel;
+                    const float4 x0 = P[j0], v0 = V[j0];
+                    const float4 x1 = P[j1], v1 = V[j1];
+                    pair(x0, v0);
+                    pair(x1, v1);
                 }
             }
             cnt = 0;
@@ -435,33 +464,30 @@ __global__ __launch_bounds__(BLOCK) void k_force(const Tile* __restrict__ tiles,
         auto consider = [&](const float4 xj, uint32_t j) {
             const float ddx = xi.x - xj.x, ddy = xi.y - xj.y, ddz = xi.z - xj.z;
             const float r2 = fmaf(ddx, ddx, fmaf(ddy, ddy, ddz * ddz));
-            if (r2 < h2 && j != s) {
-                queue[cnt][threadIdx.x] = j;
-                ++cnt;
-            }
+            queue[cnt][threadIdx.x] = j;
+            cnt += (r2 < h2 && j != s) ? 1 : 0;
         };
-        int cx, cy, cz;
-        float fx, fy, fz;
-        if (cell_coords(sc, xi.x, xi.y, xi.z, fx, fy, fz, cx, cy, cz)) {
-            for_each_row<SUB, PRUNE, BLOCK>(sc, start, fx, fy, fz, cx, cy, cz, rows + threadIdx.x,
-                                            [&](uint32_t b, uint32_t e) {
-                                                uint32_t j = b;
-                                                for (; j + 4 <= e; j += 4) {
-                                                    const float4 x0 = P[j], x1 = P[j + 1], x2 = P[j + 2], x3 = P[j + 3];
-                                                    if (__any(cnt > kQueue - 4)) flush();
-                                                    consider(x0, j);
-                                                    consider(x1, j + 1);
-                                                    consider(x2, j + 2);
-                                                    consider(x3, j + 3);
-                                                }
-                                                for (; j < e; ++j) {
-                                                    const float4 xj = P[j];
-                                                    if (__any(cnt == kQueue)) flush();
-                                                    consider(xj, j);
-                                                }
-                                            });
-        }
+        int cx = 0, cy = 0, cz = 0;
+        float fx = 0.f, fy = 0.f, fz = 0.f;
+        const bool valid = active && cell_coords(sc, xi.x, xi.y, xi.z, fx, fy, fz, cx, cy, cz);
+        gather_rows<SUB, PRUNE>(sc, start, P, lds[threadIdx.x / 64], valid, fx, fy, fz, cx, cy, cz,
+                                [&](const float4* q, uint32_t m, uint32_t j) {
+                                    for (; m >= 4u; m -= 4u, q += 4, j += 4u) {
+                                        const float4 x0 = q[0], x1 = q[1], x2 = q[2], x3 = q[3];
+                                        if (__any(cnt > kQueue - 4)) flush();
+                                        consider(x0, j);
+                                        consider(x1, j + 1u);
+                                        consider(x2, j + 2u);
+                                        consider(x3, j + 3u);
+                                    }
+                                    for (; m > 0u; --m, ++q, ++j) {
+                                        const float4 xj = *q;
+                                        if (__any(cnt == kQueue)) flush();
+                                        consider(xj, j);
+                                    }
+                                });
         flush();
+        if (!active) return;
         float f[3];
         f[0] = fmaf(sc.pres_m, ax, fmaf(sc.visc_m, bx, rhoi * sc.grav[0]));
         f[1] = fmaf(sc.pres_m, ay, fmaf(sc.visc_m, by, rhoi * sc.grav[1]));
@@ -600,7 +626,8 @@ hipError_t launch_lattice(hipStream_t s, sph_particle_t* aos, uint32_t first, co
 hipError_t launch_refresh(hipStream_t s, const DeviceState& d, bool pos_vel, bool extras)
 {
     hipLaunchKernelGGL(k_refresh, dim3(d.ntiles), dim3(kTile), 0, s, d.tiles, d.aos, d.perm, d.PB, d.VB, d.rho, d.D2,
-                       pos_vel ? 1 : 0, extras ? 1 : 0);
+                       reinterpret_cast<float*>(d.PA), reinterpret_cast<float*>(d.VA), pos_vel ? 1 : 0,
+                       extras ? 1 : 0);
     return hipGetLastError();
 }
 
@@ -657,7 +684,8 @@ template <int SUB, bool PRUNE, int BLOCK>
 hipError_t density_variant(hipStream_t s, const DeviceState& d, bool write_aos)
 {
     hipLaunchKernelGGL((k_density<SUB, PRUNE, BLOCK>), gather_grid(d), dim3(BLOCK), 0, s, d.gtiles, d.scenes, d.start,
-                       d.PA, d.rho, d.D2, d.perm, write_aos ? d.aos : nullptr, (uint32_t)d.ngtiles);
+                       d.PA, d.rho, d.D2, reinterpret_cast<float*>(d.PA), reinterpret_cast<float*>(d.VA), d.perm,
+                       write_aos ? d.aos : nullptr, (uint32_t)d.ngtiles);
     return hipGetLastError();
 }
 
@@ -666,10 +694,10 @@ hipError_t force_variant(hipStream_t s, const DeviceState& d, bool fuse)
 {
     if (fuse)
         hipLaunchKernelGGL((k_force<true, SUB, PRUNE, BLOCK>), gather_grid(d), dim3(BLOCK), 0, s, d.gtiles, d.scenes,
-                           d.start, d.PA, d.VA, d.rho, d.D2, d.perm, d.aos, d.PB, d.VB, (uint32_t)d.ngtiles);
+                           d.start, d.PA, d.VA, d.rho, d.perm, d.aos, d.PB, d.VB, d.n, (uint32_t)d.ngtiles);
     else
         hipLaunchKernelGGL((k_force<false, SUB, PRUNE, BLOCK>), gather_grid(d), dim3(BLOCK), 0, s, d.gtiles, d.scenes,
-                           d.start, d.PA, d.VA, d.rho, d.D2, d.perm, d.aos, d.PB, d.VB, (uint32_t)d.ngtiles);
+                           d.start, d.PA, d.VA, d.rho, d.perm, d.aos, d.PB, d.VB, d.n, (uint32_t)d.ngtiles);
     return hipGetLastError();
 }
 

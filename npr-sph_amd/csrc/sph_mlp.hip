@@ -40,9 +40,10 @@ constexpr int kGradRows = 16;  // rows staged in LDS per sub-chunk
 
 // MlpPackLayout: offsets of the per-lane MFMA operand images (bf16 for A,
 // float32 for the bias-initialised accumulators C).
-constexpr int kPackA1 = 0, kPackA2 = kPackA1 + 4 * 64 * 8, kPackA3 = kPackA2 + 4 * 2 * 64 * 8,
-              kPackAEnd = kPackA3 + 2 * 64 * 8;
-constexpr int kPackC1 = 0, kPackC2 = kPackC1 + 4 * 64 * 4, kPackC3 = kPackC2 + 4 * 64 * 4, kPackCEnd = kPackC3 + 64 * 4;
+constexpr int kPackA1 = 0, kPackA2 = kPackA1 + 4 * 64 * 8, kPackA3 = kPackA2 + 4 * 2 * 64 * 8;
+static_assert(kPackA3 + 2 * 64 * 8 == kMlpPackA, "weight pack layout");
+constexpr int kPackC1 = 0, kPackC2 = kPackC1 + 4 * 64 * 4, kPackC3 = kPackC2 + 4 * 64 * 4;
+static_assert(kPackC3 + 64 * 4 == kMlpPackC, "bias pack layout");
 
 __device__ __forceinline__ uint32_t scene_of(const SceneDev* __restrict__ scenes, int nscenes, uint32_t s)
 {

@@ -136,7 +136,9 @@ typedef struct sph_ctx sph_ctx;
 /* Reference defaults (Main.cpp:110-122 and the shader-baked constants). */
 void sph_defaults(sph_constants_t* c, sph_boundary_t* b, sph_fixed_t* f);
 
-/* Create a single-scene solver for n particles (n >= 1).  Any of c/b/f may be
+/* Create a single-scene solver for n particles (1 <= n <= 2^28 - 2, the total
+ * for a batch as well; the slot arrays are addressed with 32-bit byte
+ * offsets).  Any of c/b/f may be
  * NULL for the defaults.  The AoS buffer starts zeroed except pos.w = 1.
  * Replaces initOpenGL's buffer setup (Main.cpp:524-527, 632-641). */
 int sph_create(sph_ctx** out, size_t n, const sph_constants_t* c, const sph_boundary_t* b,

@@ -545,8 +545,8 @@ int sph_create_batch(sph_ctx** out, int nscenes, const size_t* counts, const sph
         }
         total += counts[s];
     }
-    if (total > 0x7fffffffull) {
-        g_create_error = "too many particles (limit 2^31-1)";
+    if (total > (1ull << 28) - 2u) {  // slot arrays (plus one sentinel) stay under 2^32 bytes of float4
+        g_create_error = "too many particles (limit 2^28-2)";
         return SPH_ERR_INVALID;
     }
     sph_ctx* c = new sph_ctx();

@@ -90,6 +90,24 @@ __device__ __forceinline__ void wave_lds_sync()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Raw buffer resources (gfx950 word 3: 32-bit format, no swizzle) give
+// 32-bit byte offsets with a scalar base, so an element index costs one
+// shift instead of 64-bit address arithmetic.  Reads at or past `count`
+// elements (the voffset range check) return zeros.  Arrays addressed this way
+// hold fewer than 2^28 float4s.
+constexpr int kBufWord3 = 0x00020000;
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t f4_rsrc(const float4* p, uint32_t count)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<float4*>(p), (short)0, (int)(count * 16u), kBufWord3);
+}
+
+// Element voff_elems + soff_elems of the buffer; soff_elems is wave-uniform.
+__device__ __forceinline__ float4 f4_load(__amdgpu_buffer_rsrc_t r, uint32_t voff_elems, uint32_t soff_elems = 0u)
+{
+    return __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(r, voff_elems * 16u, soff_elems * 16u, 0));
+}
+
 // Per-wave LDS words a gather needs: the row table of for_each_row (stride
 // 64) or the candidate stage, never both at once.
 template <int SUB>
@@ -104,19 +122,22 @@ constexpr int gather_lds_words()
 // widened by SUB is contiguous.  That range is copied into the wave's LDS
 // stage kStage candidates per round, lane k loading slots k, k+64, ...
 // (coalesced), and each lane then reads its own pruned sub-range of the round
-// from LDS.  Otherwise every lane walks its own rows through global memory
+// from LDS.  With PREFETCH the first round of row k+1 is loaded into
+// registers while row k is scanned (8 more VGPRs).  Otherwise every lane walks its own rows through global memory
 // (for_each_row).  Both ways visit the same candidates in the same order
 // (rows in z, y order, slots ascending), so results do not depend on the way
 // taken.  Lanes with valid = false visit nothing but must take part: the call
 // is made by the whole wave.
-// body(q, m, j) visits the m candidates q[0..m) whose first slot is j; q
-// points into LDS or into P.
-template <int SUB, bool PRUNE, class Body>
+// rP views P (np elements).  body(q, m, j) visits the m candidates q[0..m)
+// whose first slot is j; q points into LDS or into P.
+template <int SUB, bool PRUNE, bool PREFETCH, class Body>
 __device__ __forceinline__ void gather_rows(const SceneDev& sc, const uint32_t* __restrict__ start,
-                                            const float4* __restrict__ P, uint32_t* lds, bool valid, float fx,
-                                            float fy, float fz, int cx, int cy, int cz, Body&& body)
+                                            const float4* __restrict__ P, __amdgpu_buffer_rsrc_t rP, uint32_t* lds,
+                                            bool valid, float fx, float fy, float fz, int cx, int cy, int cz,
+                                            Body&& body)
 {
     constexpr int kRows = 2 * SUB + 1;
+    constexpr int kPer = kStage / 64;
     const int lane = (int)__lane_id();
     const int dx = sc.dims[0], dy = sc.dims[1], dz = sc.dims[2];
     const int rk = cz * dy + cy;
@@ -135,6 +156,20 @@ __device__ __forceinline__ void gather_rows(const SceneDev& sc, const uint32_t* 
     const int xlo = __builtin_amdgcn_readfirstlane(max(wave_min_i32(valid ? cx : 0x7fffffff) - SUB, 0));
     const int xhi = __builtin_amdgcn_readfirstlane(min(wave_max_i32(valid ? cx : -1) + SUB, dx - 1));
     float4* stage = reinterpret_cast<float4*>(lds);
+    auto fetch = [&](float4 (&dst)[kPer], uint32_t c0, uint32_t n) {
+#pragma unroll
+        for (int r = 0; r < kPer; ++r) {
+            const uint32_t i = (uint32_t)lane + 64u * (uint32_t)r;
+            if (i < n) dst[r] = f4_load(rP, i, c0);
+        }
+    };
+    auto put = [&](const float4 (&src)[kPer], uint32_t n) {
+#pragma unroll
+        for (int r = 0; r < kPer; ++r) {
+            const uint32_t i = (uint32_t)lane + 64u * (uint32_t)r;
+            if (i < n) stage[i] = src[r];
+        }
+    };
     const int xa = cx > SUB ? cx - SUB : 0;
     const int xb = cx < dx - SUB ? cx + SUB : dx - 1;
     const int z0 = ucz > SUB ? ucz - SUB : 0, z1 = ucz < dz - SUB ? ucz + SUB : dz - 1;
@@ -166,11 +201,28 @@ __device__ __forceinline__ void gather_rows(const SceneDev& sc, const uint32_t* 
                 }
             }
         }
+        // Slots of row k are [RB[k], RE[k]) with RE[k] >= RB[k] (cell starts
+        // are a prefix sum); each lane's [rb[k], re[k]) lies inside.
+        float4 pre[kPer];
+        fetch(pre, RB[0], min(RE[0] - RB[0], (uint32_t)kStage));
 #pragma unroll
         for (int k = 0; k < kRows; ++k) {
-            for (uint32_t c0 = RB[k]; c0 < RE[k]; c0 += kStage) {
+            const uint32_t n0 = min(RE[k] - RB[k], (uint32_t)kStage);
+            if (!PREFETCH && k > 0) fetch(pre, RB[k], n0);
+            put(pre, n0);
+            wave_lds_sync();
+            if (PREFETCH && k + 1 < kRows) fetch(pre, RB[k + 1], min(RE[k + 1] - RB[k + 1], (uint32_t)kStage));
+            {
+                const uint32_t lo = max(rb[k], RB[k]), hi = min(re[k], RB[k] + n0);
+                if (hi > lo) body(stage + (lo - RB[k]), hi - lo, lo);
+            }
+            wave_lds_sync();
+            // Further rounds of a row longer than the stage.
+            for (uint32_t c0 = RB[k] + (uint32_t)kStage; c0 < RE[k]; c0 += (uint32_t)kStage) {
                 const uint32_t n = min(RE[k] - c0, (uint32_t)kStage);
-                for (uint32_t r = (uint32_t)lane; r < n; r += 64u) stage[r] = P[c0 + r];
+                float4 more[kPer];
+                fetch(more, c0, n);
+                put(more, n);
                 wave_lds_sync();
                 const uint32_t lo = max(rb[k], c0), hi = min(re[k], c0 + n);
                 if (hi > lo) body(stage + (lo - c0), hi - lo, lo);

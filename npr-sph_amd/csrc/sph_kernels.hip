@@ -338,9 +338,10 @@ __global__ __launch_bounds__(BLOCK) void k_density(const Tile* __restrict__ tile
                                                    const uint32_t* __restrict__ start, const float4* __restrict__ P,
                                                    float* __restrict__ rho_out, float2* __restrict__ D2, float* Pw,
                                                    float* Vw, const uint32_t* __restrict__ perm,
-                                                   sph_particle_t* __restrict__ aos, uint32_t ntiles)
+                                                   sph_particle_t* __restrict__ aos, uint32_t np, uint32_t ntiles)
 {
     __shared__ uint32_t lds[BLOCK / 64][gather_lds_words<SUB>()];
+    const __amdgpu_buffer_rsrc_t rP = f4_rsrc(P, np);
     for_each_gather_tile(ntiles, [&](uint32_t ti) {
         const Tile t = tiles[ti];
         const SceneDev& sc = scenes[t.scene];
@@ -359,7 +360,7 @@ __global__ __launch_bounds__(BLOCK) void k_density(const Tile* __restrict__ tile
             return w * w * w;
         };
         const bool valid = active && cell_coords(sc, xi.x, xi.y, xi.z, fx, fy, fz, cx, cy, cz);
-        gather_rows<SUB, PRUNE>(sc, start, P, lds[threadIdx.x / 64], valid, fx, fy, fz, cx, cy, cz,
+        gather_rows<SUB, PRUNE, true>(sc, start, P, rP, lds[threadIdx.x / 64], valid, fx, fy, fz, cx, cy, cz,
                                 [&](const float4* q, uint32_t m, uint32_t) {
                                     for (; m >= 4u; m -= 4u, q += 4) {
                                         const float4 x0 = q[0], x1 = q[1], x2 = q[2], x3 = q[3];
@@ -386,13 +387,17 @@ __global__ __launch_bounds__(BLOCK) void k_density(const Tile* __restrict__ tile
     });
 }
 
-// Hits found while scanning candidates are queued per lane in LDS and then
-// processed by all lanes in lock step, so the pair-force arithmetic runs once
-// per neighbour instead of once per candidate of the slowest lane.  Every
-// candidate's index is written to the lane's next queue entry and the entry is
-// kept (count advanced) only for a hit, so the scan has no branches.
+// Hits found while scanning candidates are queued per lane in LDS (a ring of
+// kQueue entries) and then processed by all lanes in lock step, so the
+// pair-force arithmetic runs once per neighbour instead of once per candidate
+// of the slowest lane.  Every candidate's index is written to the lane's next
+// queue entry and the entry is kept (tail advanced) only for a hit, so the
+// scan has no branches.  A drain stops once fewer than kDrainLanes lanes have
+// entries left, provided no lane is within 8 entries of full: the lanes with
+// long queues keep their entries for a later drain instead of running alone.
 constexpr int kQueue = 16;
-static_assert(kQueue % 2 == 0, "the queue is drained two entries at a time");
+constexpr int kDrainLanes = 32;
+static_assert((kQueue & (kQueue - 1)) == 0 && kQueue >= 16, "queue ring size");
 
 // force_comp.glsl:35-66 over the stencil, j != i (:50-53), r < h (:57):
 //   pressure  += m(p_i+p_j)/(2 rho_j) * 45/(pi h^6) * (h-r)^2 * delta/r
@@ -401,7 +406,8 @@ static_assert(kQueue % 2 == 0, "the queue is drained two entries at a time");
 // Reads P = (x, y, z, p) and V = (vx, vy, vz, 1/rho) as left by the density
 // pass.  Slot `sentinel` (one past the last particle) holds a far-away
 // particle with p = 0 and 1/rho = 0: its pair terms are exact zeros, and it
-// fills the second entry of a queue pair that a lane does not have.
+// fills the second entry of a queue pair that a lane does not have.  P and V
+// hold sentinel + 1 elements.
 // FUSE additionally runs integrate_comp.glsl:35-81 on the new force, writes the
 // AoS record (pos, vel, force, extras.xy) and the next step's B state.
 template <bool FUSE, int SUB, bool PRUNE, int BLOCK>
@@ -414,6 +420,7 @@ __global__ __launch_bounds__(BLOCK) void k_force(const Tile* __restrict__ tiles,
 {
     __shared__ uint32_t queue[kQueue][BLOCK];
     __shared__ uint32_t lds[BLOCK / 64][gather_lds_words<SUB>()];
+    const __amdgpu_buffer_rsrc_t rP = f4_rsrc(P, sentinel + 1u), rV = f4_rsrc(V, sentinel + 1u);
     for_each_gather_tile(ntiles, [&](uint32_t ti) {
         const Tile t = tiles[ti];
         const SceneDev& sc = scenes[t.scene];
@@ -427,7 +434,7 @@ __global__ __launch_bounds__(BLOCK) void k_force(const Tile* __restrict__ tiles,
         const float h = sc.h, h2 = sc.h2;
         float ax = 0.f, ay = 0.f, az = 0.f;  // pressure sum
         float bx = 0.f, by = 0.f, bz = 0.f;  // viscosity sum
-        int cnt = 0;
+        int head = 0, tail = 0;  // this lane's queue entries are [head, tail) mod kQueue
         // One neighbour's pressure and viscosity terms.  The hardware reciprocal
         // square root of r^2 = 0 (a coincident pair) is +inf, so r = 0*inf is
         // NaN, as normalize(0) in the reference.
@@ -446,35 +453,41 @@ __global__ __launch_bounds__(BLOCK) void k_force(const Tile* __restrict__ tiles,
             by = fmaf(cv, vj.y - vi.y, by);
             bz = fmaf(cv, vj.z - vi.z, bz);
         };
-        // Drains the queue two entries per round; the loads of both entries
-        // are issued before either is used.
-        auto flush = [&]() {
-            for (int k = 0; __any(k < cnt); k += 2) {
-                if (k < cnt) {
-                    const uint32_t j0 = queue[k][threadIdx.x];
-                    const uint32_t j1 = k + 1 < cnt ? queue[k + 1][threadIdx.x] : sentinel;
-                    const float4 x0 = P[j0], v0 = V[j0];
-                    const float4 x1 = P[j1], v1 = V[j1];
+        // Processes queue entries two per round (the loads of both are issued
+        // before either is used), oldest first.  With all = false it returns
+        // with every lane at most kQueue - 8 entries full.
+        auto flush = [&](bool all) {
+            for (;;) {
+                const int n = tail - head;
+                if (all ? !__any(n > 0)
+                        : (!__any(n > kQueue - 8) && __popcll(__ballot(n > 0)) < kDrainLanes))
+                    break;
+                if (n > 0) {
+                    const uint32_t j0 = queue[head & (kQueue - 1)][threadIdx.x];
+                    const uint32_t j1 = n > 1 ? queue[(head + 1) & (kQueue - 1)][threadIdx.x] : sentinel;
+                    const float4 x0 = f4_load(rP, j0), v0 = f4_load(rV, j0);
+                    const float4 x1 = f4_load(rP, j1), v1 = f4_load(rV, j1);
                     pair(x0, v0);
                     pair(x1, v1);
+                    head += n > 1 ? 2 : 1;
                 }
             }
-            cnt = 0;
         };
+        // Needs a free entry (tail - head < kQueue).
         auto consider = [&](const float4 xj, uint32_t j) {
             const float ddx = xi.x - xj.x, ddy = xi.y - xj.y, ddz = xi.z - xj.z;
             const float r2 = fmaf(ddx, ddx, fmaf(ddy, ddy, ddz * ddz));
-            queue[cnt][threadIdx.x] = j;
-            cnt += (r2 < h2 && j != s) ? 1 : 0;
+            queue[tail & (kQueue - 1)][threadIdx.x] = j;
+            tail += (r2 < h2 && j != s) ? 1 : 0;
         };
         int cx = 0, cy = 0, cz = 0;
         float fx = 0.f, fy = 0.f, fz = 0.f;
         const bool valid = active && cell_coords(sc, xi.x, xi.y, xi.z, fx, fy, fz, cx, cy, cz);
-        gather_rows<SUB, PRUNE>(sc, start, P, lds[threadIdx.x / 64], valid, fx, fy, fz, cx, cy, cz,
+        gather_rows<SUB, PRUNE, true>(sc, start, P, rP, lds[threadIdx.x / 64], valid, fx, fy, fz, cx, cy, cz,
                                 [&](const float4* q, uint32_t m, uint32_t j) {
                                     for (; m >= 4u; m -= 4u, q += 4, j += 4u) {
                                         const float4 x0 = q[0], x1 = q[1], x2 = q[2], x3 = q[3];
-                                        if (__any(cnt > kQueue - 4)) flush();
+                                        if (__any(tail - head > kQueue - 4)) flush(false);
                                         consider(x0, j);
                                         consider(x1, j + 1u);
                                         consider(x2, j + 2u);
@@ -482,11 +495,11 @@ __global__ __launch_bounds__(BLOCK) void k_force(const Tile* __restrict__ tiles,
                                     }
                                     for (; m > 0u; --m, ++q, ++j) {
                                         const float4 xj = *q;
-                                        if (__any(cnt == kQueue)) flush();
+                                        if (__any(tail - head == kQueue)) flush(false);
                                         consider(xj, j);
                                     }
                                 });
-        flush();
+        flush(true);
         if (!active) return;
         float f[3];
         f[0] = fmaf(sc.pres_m, ax, fmaf(sc.visc_m, bx, rhoi * sc.grav[0]));
@@ -685,7 +698,7 @@ hipError_t density_variant(hipStream_t s, const DeviceState& d, bool write_aos)
 {
     hipLaunchKernelGGL((k_density<SUB, PRUNE, BLOCK>), gather_grid(d), dim3(BLOCK), 0, s, d.gtiles, d.scenes, d.start,
                        d.PA, d.rho, d.D2, reinterpret_cast<float*>(d.PA), reinterpret_cast<float*>(d.VA), d.perm,
-                       write_aos ? d.aos : nullptr, (uint32_t)d.ngtiles);
+                       write_aos ? d.aos : nullptr, d.n, (uint32_t)d.ngtiles);
     return hipGetLastError();
 }
 

@@ -234,29 +234,57 @@ __global__ __launch_bounds__(1024) void k_scan_top(uint32_t* __restrict__ bsum, 
 }
 
 // start[c] = exclusive prefix of count; counts are cleared for the next build
-// and start[n] receives the total.
+// and start[n] receives the total.  Each thread owns kScanItems consecutive
+// cells and moves them as 16-byte vectors where the whole vector is in range
+// (count and start are 256-byte aligned, kScanItems is a multiple of 4).
 __global__ __launch_bounds__(kScanThreads) void k_scan_down(uint32_t* __restrict__ count, uint32_t n,
                                                             const uint32_t* __restrict__ bsum, uint32_t* __restrict__ start)
 {
+    static_assert(kScanItems % 4 == 0, "vector width");
     const uint32_t base = blockIdx.x * (uint32_t)kScanBlock + threadIdx.x * (uint32_t)kScanItems;
+    const bool whole = base + (uint32_t)kScanItems <= n;
     uint32_t v[kScanItems];
     uint32_t sum = 0;
 #pragma unroll
-    for (int k = 0; k < kScanItems; ++k) {
+    for (int k = 0; k < kScanItems; k += 4) {
         const uint32_t i = base + (uint32_t)k;
-        v[k] = i < n ? count[i] : 0u;
-        sum += v[k];
+        if (whole) {
+            const uint4 q = *reinterpret_cast<const uint4*>(count + i);
+            v[k] = q.x;
+            v[k + 1] = q.y;
+            v[k + 2] = q.z;
+            v[k + 3] = q.w;
+        } else {
+#pragma unroll
+            for (int e = 0; e < 4; ++e) v[k + e] = i + (uint32_t)e < n ? count[i + (uint32_t)e] : 0u;
+        }
     }
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) sum += v[k];
     uint32_t total;
     uint32_t run = bsum[blockIdx.x] + block_excl_scan<kScanThreads>(sum, total);
+    uint32_t out[kScanItems];
 #pragma unroll
     for (int k = 0; k < kScanItems; ++k) {
-        const uint32_t i = base + (uint32_t)k;
-        if (i < n) {
-            start[i] = run;
-            count[i] = 0u;
-            run += v[k];
-            if (i == n - 1u) start[n] = run;
+        out[k] = run;
+        run += v[k];
+    }
+    if (whole) {
+#pragma unroll
+        for (int k = 0; k < kScanItems; k += 4) {
+            *reinterpret_cast<uint4*>(start + base + k) = make_uint4(out[k], out[k + 1], out[k + 2], out[k + 3]);
+            *reinterpret_cast<uint4*>(count + base + k) = make_uint4(0u, 0u, 0u, 0u);
+        }
+        if (base + (uint32_t)kScanItems == n) start[n] = run;
+    } else {
+#pragma unroll
+        for (int k = 0; k < kScanItems; ++k) {
+            const uint32_t i = base + (uint32_t)k;
+            if (i < n) {
+                start[i] = out[k];
+                count[i] = 0u;
+                if (i == n - 1u) start[n] = out[k] + v[k];
+            }
         }
     }
 }

@@ -251,10 +251,13 @@ def main(argv=None):
     roofline = None
     if algo is not None:
         achieved = algo * n / (dom_ms * 1e-3) / 1e9
+        traffic = measured_traffic(args.workload, dom, tuning)
         roofline = {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(args.workload, dom, tuning),
+                    "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                    "traffic_gbs": None if traffic is None else traffic / (dom_ms * 1e-3) / 1e9,
                     "algo_bytes_per_particle": algo, "launch_ms": dom_ms,
-                    "binding_roof": "VALU issue (gathers are instruction-bound; see DESIGN.md section 5)"}
+                    "binding_roof": "VALU issue: the neighbour gathers are instruction-bound (about 10 VALU "
+                                    "instructions per candidate pair), with HBM mostly idle"}
     m = sim.metrics(0)
     sim.close()
 

@@ -206,7 +206,7 @@ def main(argv=None):
     ap.add_argument("--copy-probe", type=int, default=0, help="MiB: run only a plain copy kernel")
     ap.add_argument("--tune", default="auto",
                     help="'auto' (autotune on the warmed state, untimed), 'default', or cells_per_h,prune,block,bpx")
-    ap.add_argument("--tune-steps", type=int, default=5)
+    ap.add_argument("--tune-steps", type=int, default=10)
     ap.add_argument("--no-learned", action="store_true", help="skip the learned coarse-step measurement")
     args = ap.parse_args(argv)
     if args.gpus != 1:

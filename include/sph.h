@@ -211,8 +211,9 @@ int sph_download_grid(sph_ctx* ctx, uint32_t* cell_start, size_t ncell_entries, 
 int sph_metrics(sph_ctx* ctx, int scene, sph_metrics_t* out);
 
 /* Gather tuning (no reference counterpart).  sph_autotune times every
- * supported setting for `trial_steps` steps from the current state, restores
- * that state, applies the fastest setting and reports it. */
+ * supported setting for `trial_steps` steps from the current state, re-times
+ * the four fastest for 4*trial_steps steps, restores that state, applies the
+ * fastest setting and reports it with its per-step milliseconds. */
 int sph_get_tuning(const sph_ctx* ctx, sph_tuning_t* out);
 int sph_set_tuning(sph_ctx* ctx, const sph_tuning_t* t);
 int sph_autotune(sph_ctx* ctx, int trial_steps, sph_tuning_t* best, double* best_ms_per_step);

@@ -965,8 +965,28 @@ int sph_autotune(sph_ctx* c, int trial_steps, sph_tuning_t* best, double* best_m
     hipEvent_t e0 = nullptr, e1 = nullptr;
     SPH_HIP(c, hipEventCreate(&e0));
     SPH_HIP(c, hipEventCreate(&e1));
-    sph_tuning_t win = start_tuning;
-    float win_ms = 1e30f;
+    // Time `steps` steps of tuning t from the snapshot (after one untimed step
+    // that also captures the graphs); per-step milliseconds in *ms.
+    auto trial = [&](const sph_tuning_t& t, int steps, float* ms) -> int {
+        int r = sph_set_tuning(c, &t);
+        if (r != SPH_OK) return r;
+        if (hipMemcpyAsync(c->d.aos, c->d_backup, sizeof(sph_particle_t) * c->n, hipMemcpyDeviceToDevice,
+                           c->stream) != hipSuccess)
+            return hip_fail(c, hipErrorUnknown, "restore snapshot");
+        invalidate_state(c);
+        if ((r = sph_step(c, 1)) != SPH_OK) return r;
+        if (hipEventRecord(e0, c->stream) != hipSuccess) return hip_fail(c, hipErrorUnknown, "event");
+        if ((r = sph_step(c, steps)) != SPH_OK) return r;
+        float total = 0.f;
+        if (hipEventRecord(e1, c->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+            hipEventElapsedTime(&total, e0, e1) != hipSuccess)
+            return hip_fail(c, hipErrorUnknown, "autotune timing");
+        *ms = total / (float)steps;
+        return SPH_OK;
+    };
+    // Screen every setting with trial_steps steps, then re-time the four
+    // fastest with four times as many and keep the fastest of those.
+    std::vector<std::pair<float, sph_tuning_t>> screened;
     const int32_t bpx_options[2] = {0, 64};
     for (int32_t sub = 1; sub <= 3 && rc == SPH_OK; ++sub)
         for (int32_t prune = 0; prune <= 1 && rc == SPH_OK; ++prune)
@@ -974,27 +994,21 @@ int sph_autotune(sph_ctx* c, int trial_steps, sph_tuning_t* best, double* best_m
                 for (int32_t bpx : bpx_options) {
                     if (rc != SPH_OK) break;
                     const sph_tuning_t t{sub, prune, block, bpx};
-                    if ((rc = sph_set_tuning(c, &t)) != SPH_OK) break;
-                    if (hipMemcpyAsync(c->d.aos, c->d_backup, sizeof(sph_particle_t) * c->n, hipMemcpyDeviceToDevice,
-                                       c->stream) != hipSuccess) {
-                        rc = hip_fail(c, hipErrorUnknown, "restore snapshot");
-                        break;
-                    }
-                    invalidate_state(c);
-                    if ((rc = sph_step(c, 1)) != SPH_OK) break;  // warm-up and graph capture
-                    if (hipEventRecord(e0, c->stream) != hipSuccess) { rc = hip_fail(c, hipErrorUnknown, "event"); break; }
-                    if ((rc = sph_step(c, trial_steps)) != SPH_OK) break;
                     float ms = 0.f;
-                    if (hipEventRecord(e1, c->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
-                        hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
-                        rc = hip_fail(c, hipErrorUnknown, "autotune timing");
-                        break;
-                    }
-                    if (ms < win_ms) {
-                        win_ms = ms;
-                        win = t;
-                    }
+                    if ((rc = trial(t, trial_steps, &ms)) == SPH_OK) screened.emplace_back(ms, t);
                 }
+    std::stable_sort(screened.begin(), screened.end(),
+                     [](const auto& a, const auto& b) { return a.first < b.first; });
+    sph_tuning_t win = start_tuning;
+    float win_ms = 1e30f;
+    for (size_t k = 0; k < screened.size() && k < 4 && rc == SPH_OK; ++k) {
+        float ms = 0.f;
+        if ((rc = trial(screened[k].second, 4 * trial_steps, &ms)) != SPH_OK) break;
+        if (ms < win_ms) {
+            win_ms = ms;
+            win = screened[k].second;
+        }
+    }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     c->timing = was_timing;
@@ -1010,7 +1024,7 @@ int sph_autotune(sph_ctx* c, int trial_steps, sph_tuning_t* best, double* best_m
     }
     if (rc != SPH_OK) return rc;
     if (best) *best = win;
-    if (best_ms) *best_ms = (double)win_ms / trial_steps;
+    if (best_ms) *best_ms = (double)win_ms;
     return SPH_OK;
 }
 

@@ -503,6 +503,27 @@ __global__ __launch_bounds__(256) void k_coarse_mlp_fp32(const Tile* __restrict_
 // weights (k_mlp_pack).
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef _Float16 f16x2 __attribute__((ext_vector_type(2)));
+
+// gelu_poly of two values in packed FP16 arithmetic, rounded to BF16 for the
+// next MFMA.  Written with c = clamp(x, -3, 3):
+//   0.5c + q(c^2) + max(x - c, 0),
+// which equals gelu_poly for every finite x (q(9) - 1.5 is the same offset
+// beyond the knee in both) and stays finite for x = -inf.  FP16 carries 11
+// significant bits against BF16's 8, so the result is the BF16 rounding of
+// gelu_poly to within one BF16 step.
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ bf16x2 gelu2_bf16(float a, float b)
+{
+    const f16x2 x = {(_Float16)a, (_Float16)b};
+    const f16x2 knee = (_Float16)kGeluKnee;
+    const f16x2 c = __builtin_elementwise_min(__builtin_elementwise_max(x, -knee), knee);
+    const f16x2 sq = c * c;
+    const f16x2 q = ((((f16x2)(_Float16)kGeluC3 * sq + (_Float16)kGeluC2) * sq + (_Float16)kGeluC1) * sq +
+                     (_Float16)kGeluC0) * sq;
+    const f16x2 g = (f16x2)(_Float16)0.5f * c + q + __builtin_elementwise_max(x - c, (f16x2)(_Float16)0.f);
+    return bf16x2{(__bf16)(float)g.x, (__bf16)(float)g.y};
+}
 
 __host__ __device__ __forceinline__ int kslot_unit(int kslot)
 {
@@ -628,7 +649,11 @@ __global__ __launch_bounds__(256) void k_coarse_mlp_mfma(const SceneDev* __restr
 #pragma unroll
                     for (int mm = 0; mm < 2; ++mm)
 #pragma unroll
-                        for (int e = 0; e < 4; ++e) b2[ks][4 * mm + e] = (__bf16)gelu_poly(h1[2 * ks + mm][e]);
+                        for (int e = 0; e < 4; e += 2) {
+                            const bf16x2 g = gelu2_bf16(h1[2 * ks + mm][e], h1[2 * ks + mm][e + 1]);
+                            b2[ks][4 * mm + e] = g.x;
+                            b2[ks][4 * mm + e + 1] = g.y;
+                        }
                 f32x4 h2[4];
 #pragma unroll
                 for (int m = 0; m < 4; ++m) {
@@ -641,7 +666,11 @@ __global__ __launch_bounds__(256) void k_coarse_mlp_mfma(const SceneDev* __restr
 #pragma unroll
                     for (int mm = 0; mm < 2; ++mm)
 #pragma unroll
-                        for (int e = 0; e < 4; ++e) b3[ks][4 * mm + e] = (__bf16)gelu_poly(h2[2 * ks + mm][e]);
+                        for (int e = 0; e < 4; e += 2) {
+                            const bf16x2 g = gelu2_bf16(h2[2 * ks + mm][e], h2[2 * ks + mm][e + 1]);
+                            b3[ks][4 * mm + e] = g.x;
+                            b3[ks][4 * mm + e + 1] = g.y;
+                        }
                 f32x4 o = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a3[0], b3[0], c3, 0, 0, 0);
                 o = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a3[1], b3[1], o, 0, 0, 0);
                 if (lane < 16 && valid) {

@@ -159,6 +159,34 @@ def test_edge_cases_vs_oracle(pkg, orc, gpu_lib):
     assert s1["extras"][2, 0] == 0.0 and s1["extras"][0, 0] > 0 and s1["extras"][1, 0] > GOLD["isolated"]["rho"]
 
 
+def test_force_on_uploaded_extras(pkg, orc, gpu_lib):
+    """sph_force with no sph_rho_pres before it takes rho and p from the uploaded
+    extras (force_comp.glsl reads them from the buffer); here they are not the
+    state's own density, so a stale or missing copy would show."""
+    sc = pkg.scenes.dam10k()
+    p0 = _lattice(orc, sc)
+    ob = orc.make_boundary(sc.lower, sc.upper)
+    rng = np.random.default_rng(5)
+    with pkg.Simulation(sc.count, boundaries=pkg.sph.make_boundary(sc.lower, sc.upper)) as sim:
+        sim.upload(p0)
+        sim.step(60)
+        st = sim.download()
+        st["extras"][:, 0] = rng.uniform(2e4, 6e4, sc.count).astype(np.float32)
+        st["extras"][:, 1] = rng.uniform(0.0, 1e8, sc.count).astype(np.float32)
+        sim.upload(st)
+        sim.force()
+        got = sim.download()
+        # and after a density pass the packed copies follow the new density
+        sim.upload(st)
+        sim.rho_pres()
+        s1 = sim.download()
+        sim.force()
+        s2 = sim.download()
+    assert np.array_equal(got["extras"], st["extras"])
+    check_force(orc, st, got, ob, grid=False)
+    check_force(orc, s1, s2, ob, grid=False)
+
+
 def test_set_constants_regrids(pkg, orc, gpu_lib):
     """The reference's smoothing slider (Main.cpp:244) changes h; the grid follows."""
     sc = pkg.scenes.dam10k()

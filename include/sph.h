@@ -267,6 +267,7 @@ typedef struct sph_coarse_stats {
     int64_t fallbacks;
     double last_drift;
     double max_drift;
+    int64_t pairs;         /* neighbour pairs the network evaluated in the last message pass */
 } sph_coarse_stats_t;
 
 /* Load / read the network (float32 master weights) and its input scales. */

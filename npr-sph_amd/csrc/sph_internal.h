@@ -96,6 +96,7 @@ struct MlpWork {
     float* partial = nullptr;  // kMlpParams per gradient block
     __bf16* pack_a = nullptr;  // MFMA operand images (k_mlp_pack)
     float* pack_c = nullptr;
+    unsigned long long* pairs = nullptr;  // pairs evaluated by the last MFMA message pass
     bool pack_stale = true;    // params changed since the last pack
 };
 

@@ -575,9 +575,11 @@ __global__ __launch_bounds__(256) void k_coarse_mlp_mfma(const SceneDev* __restr
                                                          const float4* __restrict__ P, const float4* __restrict__ V,
                                                          const float* __restrict__ rho, const float2* __restrict__ D2,
                                                          const __bf16* __restrict__ pa, const float* __restrict__ pc,
-                                                         MlpNorm nm, uint32_t n, float* __restrict__ pred)
+                                                         MlpNorm nm, uint32_t n, float* __restrict__ pred,
+                                                         unsigned long long* __restrict__ pair_count)
 {
     __shared__ uint32_t hits_all[4][kHitCap];
+    uint32_t wave_pairs = 0;  // pairs this wave evaluated, added to *pair_count at the end
     const int lane = (int)(threadIdx.x & 63);
     const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     uint32_t* hits = hits_all[wv];
@@ -721,6 +723,7 @@ __global__ __launch_bounds__(256) void k_coarse_mlp_mfma(const SceneDev* __restr
                                                                             __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
                         if (hit) hits[posn] = j;
                         nh += __popcll(mask);
+                        wave_pairs += (uint32_t)__popcll(mask);
                         if (nh > kHitCap - 64) nh = run_tiles(nh, false);
                     }
                 }
@@ -740,6 +743,7 @@ __global__ __launch_bounds__(256) void k_coarse_mlp_mfma(const SceneDev* __restr
             pred[s * 3 + 2] = acc2;
         }
     }
+    if (lane == 0 && wave_pairs != 0u) atomicAdd(pair_count, (unsigned long long)wave_pairs);
 }
 
 // pred of slot s scattered to AoS order (diagnostics).
@@ -846,10 +850,12 @@ hipError_t launch_coarse_mlp(hipStream_t st, const DeviceState& d, int nscenes, 
 {
     const int sub = d.tuning.cells_per_h;
     if (path == 1) {
+        const hipError_t e = launch_fill_u32(st, reinterpret_cast<uint32_t*>(w.pairs), 2u, 0u);
+        if (e != hipSuccess) return e;
         const dim3 grid(std::min<uint32_t>((d.n + 3u) / 4u, 256u * 64u)), block(256);
 #define NPRSPH_MFMA(S)                                                                                             \
     hipLaunchKernelGGL(k_coarse_mlp_mfma<S>, grid, block, 0, st, d.scenes, nscenes, d.start, d.PA, d.VA, d.rho, d.D2, \
-                       w.pack_a, w.pack_c, w.norm, d.n, w.pred)
+                       w.pack_a, w.pack_c, w.norm, d.n, w.pred, w.pairs)
         if (sub == 1)
             NPRSPH_MFMA(1);
         else if (sub == 2)

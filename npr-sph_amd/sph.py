@@ -98,7 +98,8 @@ class CoarseOpts(C.Structure):
 
 class CoarseStats(C.Structure):
     _fields_ = [("coarse_steps", C.c_int64), ("plain_steps", C.c_int64), ("judged", C.c_int64),
-                ("fallbacks", C.c_int64), ("last_drift", C.c_double), ("max_drift", C.c_double)]
+                ("fallbacks", C.c_int64), ("last_drift", C.c_double), ("max_drift", C.c_double),
+                ("pairs", C.c_int64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -45,7 +45,7 @@ def test_struct_layouts(pkg):
     assert C.sizeof(s.GridInfo) == 56
     assert C.sizeof(s.Tuning) == 16
     assert C.sizeof(s.MlpNorm) == 16 and C.sizeof(s.TrainOpts) == 28 and C.sizeof(s.TrainResult) == 24
-    assert C.sizeof(s.CoarseOpts) == 16 and C.sizeof(s.CoarseStats) == 48
+    assert C.sizeof(s.CoarseOpts) == 16 and C.sizeof(s.CoarseStats) == 56
     assert C.sizeof(s.Metrics) == 56
     assert s.PARTICLE_DTYPE.fields["pos"][1] == 0 and s.PARTICLE_DTYPE.fields["extras"][1] == 48
 

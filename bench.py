@@ -195,17 +195,18 @@ def learned_form(pkg, scenes, warmup, fine_steps, plain_ms_per_step, weights_pat
     }
 
 
-def measured_traffic(workload, kernel, tuning):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE /
-    WRITE_SIZE summary (calibrated by a plain copy), when it was measured for
-    this workload and tuning; otherwise None."""
+def profile_record(workload, kernel, tuning):
+    """The committed rocprofv3 record of `kernel` for this workload and tuning
+    (profiles/traffic_<workload>.json), or None: HBM bytes per launch from
+    FETCH_SIZE / WRITE_SIZE calibrated by a plain copy, and the VALU-busy
+    fraction from SQ_ACTIVE_INST_VALU / SQ_BUSY_CYCLES."""
     path = os.path.join(ROOT, "profiles", f"traffic_{workload}.json")
     if not os.path.exists(path):
         return None
     rec = json.load(open(path))
     if rec.get("kernel") != kernel or rec.get("tuning") != tuning:
         return None
-    return rec["traffic_bytes_per_launch"]
+    return rec
 
 
 def copy_probe(mbytes: int, reps: int = 20):
@@ -281,10 +282,12 @@ def main(argv=None):
     roofline = None
     if algo is not None:
         achieved = algo * n / (dom_ms * 1e-3) / 1e9
-        traffic = measured_traffic(args.workload, dom, tuning)
+        prof = profile_record(args.workload, dom, tuning)
+        traffic = None if prof is None else prof["traffic_bytes_per_launch"]
         roofline = {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                     "traffic_gbs": None if traffic is None else traffic / (dom_ms * 1e-3) / 1e9,
+                    "valu_busy": None if prof is None else prof.get("valu_busy"),
                     "algo_bytes_per_particle": algo, "launch_ms": dom_ms,
                     "binding_roof": "VALU issue: the neighbour gathers are instruction-bound (about 10 VALU "
                                     "instructions per candidate pair), with HBM mostly idle"}

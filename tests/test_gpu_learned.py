@@ -269,3 +269,26 @@ def test_trained_network_beats_no_network(pkg, orc, gpu_lib):
 
     l_net, l_sph, l_zero = loss(pred), loss(sph_acc / A), loss(np.zeros_like(tgt))
     assert l_net < 0.5 * l_zero and l_net < 0.1 * l_sph, (l_net, l_sph, l_zero)
+
+
+def test_coarse_step_counts_every_pair(pkg, orc, gpu_lib):
+    """sph_coarse_stats_t.pairs equals the number of ordered neighbour pairs
+    (r < h, j != i) of the state the message pass ran on, counted by brute force
+    in float32 on the CPU."""
+    params = np.random.default_rng(3).normal(0, 0.05, pkg.sph.MLP_PARAMS).astype(np.float32)
+    sim, sc = _scene(pkg, orc)
+    with sim:
+        sim.mlp_set(params, _norm(pkg))
+        st = sim.download()
+        stats = sim.coarse_step(1, use_network=True)
+    pos = st["pos"][:, :3].astype(np.float32)
+    h2 = np.float32(orc.smoothing_length()) * np.float32(orc.smoothing_length())
+    total, margin = 0, np.inf
+    for i in range(len(pos)):
+        d = (pos[i][None, :] - pos).astype(np.float32)
+        r2 = (d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1] + d[:, 2] * d[:, 2]).astype(np.float32)
+        r2[i] = np.inf
+        total += int(np.count_nonzero(r2 < h2))
+        margin = min(margin, float(np.min(np.abs(r2.astype(np.float64) / float(h2) - 1.0))))
+    assert margin > 1e-6  # no pair so close to r = h that float32 rounding could decide it
+    assert stats.pairs == total and total > 0

@@ -207,6 +207,20 @@ int sph_grid_info(const sph_ctx* ctx, int scene, sph_grid_info_t* out);
 size_t sph_total_cells(const sph_ctx* ctx);
 int sph_download_grid(sph_ctx* ctx, uint32_t* cell_start, size_t ncell_entries, uint32_t* perm, size_t n);
 
+/* On-disk snapshot of the particle buffer (no reference counterpart; the
+ * reference keeps its state in the GL buffer only).  The file is a 64-byte
+ * little-endian header:
+ *   0  char[8]  "NPRSPHv1"       8  uint32 header bytes (64)
+ *   12 uint32   record bytes (64) 16 uint64 particle count
+ *   24 uint32   scene count       28 uint32 reserved (0)
+ *   32 uint64   frame (the caller's tag)   40..63 reserved (0)
+ * then one uint64 particle count per scene, then the sph_particle_t records
+ * in original index order: pos at offset 0, stride 64, the layout a renderer
+ * binds (Main.cpp:533-535).  sph_load reads a file with the same per-scene
+ * counts and replaces the state (as sph_upload); frame may be NULL. */
+int sph_save(sph_ctx* ctx, const char* path, uint64_t frame);
+int sph_load(sph_ctx* ctx, const char* path, uint64_t* frame);
+
 /* Reductions over scene `scene` of the AoS state (waits). */
 int sph_metrics(sph_ctx* ctx, int scene, sph_metrics_t* out);
 

@@ -750,6 +750,86 @@ int sph_download(sph_ctx* c, sph_particle_t* host, size_t first, size_t n)
     return SPH_OK;
 }
 
+namespace {
+
+constexpr char kSnapMagic[8] = {'N', 'P', 'R', 'S', 'P', 'H', 'v', '1'};
+
+struct SnapHeader {
+    char magic[8];
+    uint32_t header_bytes;
+    uint32_t record_bytes;
+    uint64_t count;
+    uint32_t nscenes;
+    uint32_t reserved0;
+    uint64_t frame;
+    uint8_t reserved[24];
+};
+static_assert(sizeof(SnapHeader) == 64, "snapshot header layout");
+
+}  // namespace
+
+int sph_save(sph_ctx* c, const char* path, uint64_t frame)
+{
+    if (!c) return SPH_ERR_INVALID;
+    if (!path) return fail(c, SPH_ERR_INVALID, "path is null");
+    std::vector<sph_particle_t> host(c->n);
+    int rc = sph_download(c, host.data(), 0, c->n);
+    if (rc != SPH_OK) return rc;
+    SnapHeader h{};
+    std::memcpy(h.magic, kSnapMagic, sizeof(kSnapMagic));
+    h.header_bytes = sizeof(SnapHeader);
+    h.record_bytes = sizeof(sph_particle_t);
+    h.count = c->n;
+    h.nscenes = (uint32_t)c->nscenes;
+    h.frame = frame;
+    std::vector<uint64_t> counts(c->count.begin(), c->count.end());
+    FILE* f = std::fopen(path, "wb");
+    if (!f) return fail(c, SPH_ERR_INVALID, std::string("cannot open for writing: ") + path);
+    bool ok = std::fwrite(&h, sizeof(h), 1, f) == 1 &&
+              std::fwrite(counts.data(), sizeof(uint64_t), counts.size(), f) == counts.size() &&
+              std::fwrite(host.data(), sizeof(sph_particle_t), host.size(), f) == host.size();
+    ok = (std::fclose(f) == 0) && ok;
+    if (!ok) return fail(c, SPH_ERR_INVALID, std::string("write failed: ") + path);
+    return SPH_OK;
+}
+
+int sph_load(sph_ctx* c, const char* path, uint64_t* frame)
+{
+    if (!c) return SPH_ERR_INVALID;
+    if (!path) return fail(c, SPH_ERR_INVALID, "path is null");
+    FILE* f = std::fopen(path, "rb");
+    if (!f) return fail(c, SPH_ERR_INVALID, std::string("cannot open: ") + path);
+    SnapHeader h{};
+    std::vector<uint64_t> counts;
+    std::vector<sph_particle_t> host;
+    const char* why = nullptr;
+    if (std::fread(&h, sizeof(h), 1, f) != 1 || std::memcmp(h.magic, kSnapMagic, sizeof(kSnapMagic)) != 0)
+        why = "not a snapshot file";
+    else if (h.header_bytes != sizeof(SnapHeader) || h.record_bytes != sizeof(sph_particle_t))
+        why = "unsupported snapshot layout";
+    else if (h.count != c->n || h.nscenes != (uint32_t)c->nscenes)
+        why = "snapshot particle or scene count differs from the context";
+    if (!why) {
+        counts.resize(h.nscenes);
+        if (std::fread(counts.data(), sizeof(uint64_t), counts.size(), f) != counts.size())
+            why = "truncated snapshot";
+        else
+            for (int s = 0; s < c->nscenes; ++s)
+                if (counts[s] != (uint64_t)c->count[s]) why = "snapshot scene counts differ from the context";
+    }
+    if (!why) {
+        host.resize(c->n);
+        if (std::fread(host.data(), sizeof(sph_particle_t), host.size(), f) != host.size()) why = "truncated snapshot";
+    }
+    std::fclose(f);
+    if (why) return fail(c, SPH_ERR_INVALID, why);
+    int rc = sph_upload(c, host.data(), 0, c->n);
+    if (rc != SPH_OK) return rc;
+    SPH_HIP(c, hipStreamSynchronize(c->stream));
+    if (frame) *frame = h.frame;
+    return SPH_OK;
+}
+
 int sph_init_lattice(sph_ctx* c, int scene, const sph_lattice_t* spec)
 {
     if (!c) return SPH_ERR_INVALID;

@@ -33,7 +33,46 @@ EXPORTS = [
     "sph_device_buffer", "sph_stream", "sph_synchronize", "sph_grid_info", "sph_total_cells", "sph_download_grid",
     "sph_metrics", "sph_set_timing", "sph_kernel_times", "sph_get_tuning", "sph_set_tuning", "sph_autotune",
     "sph_mlp_set", "sph_mlp_get", "sph_mlp_train", "sph_mlp_predict", "sph_coarse_step", "sph_copy_state",
+    "sph_save", "sph_load",
 ]
+
+# Snapshot files (sph_save / sph_load): 64-byte header, per-scene counts, records.
+SNAPSHOT_MAGIC = b"NPRSPHv1"
+SNAPSHOT_HEADER = np.dtype([("magic", "S8"), ("header_bytes", "<u4"), ("record_bytes", "<u4"), ("count", "<u8"),
+                            ("nscenes", "<u4"), ("reserved0", "<u4"), ("frame", "<u8"), ("reserved", "V24")])
+assert SNAPSHOT_HEADER.itemsize == 64
+
+
+def write_snapshot(path: str, particles: np.ndarray, counts=None, frame: int = 0) -> None:
+    """Write particles (PARTICLE_DTYPE, original index order) in the sph_save format."""
+    particles = np.ascontiguousarray(particles, dtype=PARTICLE_DTYPE)
+    counts = np.asarray([len(particles)] if counts is None else counts, dtype="<u8")
+    if int(counts.sum()) != len(particles):
+        raise ValueError("scene counts must add up to the particle count")
+    h = np.zeros(1, dtype=SNAPSHOT_HEADER)
+    h["magic"], h["header_bytes"], h["record_bytes"] = SNAPSHOT_MAGIC, 64, PARTICLE_DTYPE.itemsize
+    h["count"], h["nscenes"], h["frame"] = len(particles), len(counts), frame
+    with open(path, "wb") as f:
+        f.write(h.tobytes())
+        f.write(counts.tobytes())
+        f.write(particles.tobytes())
+
+
+def read_snapshot(path: str):
+    """(particles, per-scene counts, frame) of a file written by sph_save,
+    without the library: what a renderer or an offline tool reads."""
+    with open(path, "rb") as f:
+        h = np.frombuffer(f.read(64), dtype=SNAPSHOT_HEADER)
+        if len(h) != 1 or h["magic"][0] != SNAPSHOT_MAGIC:
+            raise ValueError(f"{path}: not a snapshot file")
+        if h["header_bytes"][0] != 64 or h["record_bytes"][0] != PARTICLE_DTYPE.itemsize:
+            raise ValueError(f"{path}: unsupported snapshot layout")
+        counts = np.frombuffer(f.read(8 * int(h["nscenes"][0])), dtype="<u8")
+        n = int(h["count"][0])
+        particles = np.frombuffer(f.read(n * PARTICLE_DTYPE.itemsize), dtype=PARTICLE_DTYPE)
+        if len(particles) != n or len(counts) != int(h["nscenes"][0]):
+            raise ValueError(f"{path}: truncated snapshot")
+    return particles.copy(), counts.copy(), int(h["frame"][0])
 
 MLP_PARAMS = 4995
 
@@ -154,6 +193,8 @@ def load_library(path: str | None = None):
         "sph_grid_info": (C.c_int, [vp, C.c_int, C.POINTER(GridInfo)]),
         "sph_total_cells": (C.c_size_t, [vp]),
         "sph_download_grid": (C.c_int, [vp, vp, C.c_size_t, vp, C.c_size_t]),
+        "sph_save": (C.c_int, [vp, C.c_char_p, C.c_uint64]),
+        "sph_load": (C.c_int, [vp, C.c_char_p, C.POINTER(C.c_uint64)]),
         "sph_metrics": (C.c_int, [vp, C.c_int, C.POINTER(Metrics)]),
         "sph_set_timing": (C.c_int, [vp, C.c_int]),
         "sph_kernel_times": (C.c_int, [vp, C.c_int, vp, vp, vp]),
@@ -324,6 +365,16 @@ class Simulation:
         perm = np.empty(self.n, dtype=np.uint32)
         self._check(self._L.sph_download_grid(self._h, start.ctypes.data, nc, perm.ctypes.data, self.n))
         return start, perm
+
+    def save(self, path: str, frame: int = 0):
+        """Write the particle buffer to `path` (sph_save)."""
+        self._check(self._L.sph_save(self._h, os.fsencode(path), int(frame)))
+
+    def load(self, path: str) -> int:
+        """Replace the state with a file written by save(); returns its frame (sph_load)."""
+        fr = C.c_uint64()
+        self._check(self._L.sph_load(self._h, os.fsencode(path), C.byref(fr)))
+        return fr.value
 
     def metrics(self, scene: int = 0) -> Metrics:
         m = Metrics()

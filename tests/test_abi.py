@@ -9,6 +9,7 @@ import os
 import re
 
 import numpy as np
+import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 HEADER = os.path.join(ROOT, "include", "sph.h")
@@ -72,3 +73,24 @@ def test_null_context_calls_fail_cleanly(gpu_lib):
     vp = C.c_void_p()
     assert L.sph_create_batch(C.byref(vp), 0, None, None, None, None, None) == -1
     assert b"scene" in L.sph_create_error()
+
+
+def test_snapshot_format_round_trip(pkg, tmp_path):
+    """The snapshot reader and writer agree with each other and with the header
+    layout sph.h documents (no device work)."""
+    sph = pkg.sph
+    rng = np.random.default_rng(1)
+    p = np.zeros(37, dtype=sph.PARTICLE_DTYPE)
+    p["pos"] = rng.normal(size=(37, 4)).astype(np.float32)
+    p["vel"] = rng.normal(size=(37, 4)).astype(np.float32)
+    path = str(tmp_path / "s.snap")
+    sph.write_snapshot(path, p, counts=[30, 7], frame=123)
+    raw = open(path, "rb").read()
+    assert raw[:8] == b"NPRSPHv1" and len(raw) == 64 + 2 * 8 + 37 * 64
+    assert int.from_bytes(raw[16:24], "little") == 37 and int.from_bytes(raw[24:28], "little") == 2
+    assert int.from_bytes(raw[32:40], "little") == 123
+    assert raw[64 + 16:64 + 16 + 16] == p["pos"][0].tobytes()  # pos at offset 0 of the first record
+    q, counts, frame = sph.read_snapshot(path)
+    assert q.tobytes() == p.tobytes() and list(counts) == [30, 7] and frame == 123
+    with pytest.raises(ValueError):
+        sph.write_snapshot(path, p, counts=[30, 8])

@@ -247,3 +247,36 @@ def test_metrics_match_numpy(pkg, orc, gpu_lib):
     assert m2.nonfinite == 1 and np.isfinite(m2.kinetic)
     assert m2.kinetic == pytest.approx(0.5 * float(np.float32(0.02)) * (v[keep] ** 2).sum(), rel=1e-10)
     assert np.allclose(m2.com, st["pos"][keep, :3].astype(np.float64).mean(0), rtol=1e-10)
+
+
+def test_snapshot_save_load_resume(pkg, gpu_lib, tmp_path):
+    """sph_save writes the documented format (byte for byte what the numpy
+    writer produces from the downloaded state); sph_load into a fresh context
+    resumes the run bit for bit; mismatched or damaged files are refused."""
+    sph = pkg.sph
+    scenes = [s for s in pkg.scenes.batch256() if s.count < 8000][:2]
+    bnds = [sph.make_boundary(s.lower, s.upper) for s in scenes]
+    counts = [s.count for s in scenes]
+    path, ref = str(tmp_path / "a.snap"), str(tmp_path / "b.snap")
+    with pkg.Simulation(counts, boundaries=bnds) as a:
+        for i, s in enumerate(scenes):
+            a.init_lattice(sph.make_lattice(**s.lattice_args()), scene=i)
+        a.step(25)
+        a.save(path, frame=25)
+        sph.write_snapshot(ref, a.download(), counts=counts, frame=25)
+        assert open(path, "rb").read() == open(ref, "rb").read()
+        a.step(20)
+        want = a.download()
+    with pkg.Simulation(counts, boundaries=bnds) as b:
+        assert b.load(path) == 25
+        b.step(20)
+        assert b.download().tobytes() == want.tobytes()
+        with pytest.raises(pkg.SphError):
+            b.load(str(tmp_path / "missing.snap"))
+        open(str(tmp_path / "short.snap"), "wb").write(open(path, "rb").read()[:-64])
+        with pytest.raises(pkg.SphError):
+            b.load(str(tmp_path / "short.snap"))
+        b.step(1)  # still usable
+    with pkg.Simulation(sum(counts)) as c:  # one scene of the same total size
+        with pytest.raises(pkg.SphError):
+            c.load(path)

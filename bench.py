@@ -43,6 +43,13 @@ STEP_BYTES = 20 + 44 + 64  # the three reference passes, 128 B per particle-step
 # Learned message pass: flops of the pair MLP 9 -> 64 -> 64 -> 3 per neighbour
 # pair (multiply-adds counted as 2), against MI355X's dense BF16 MFMA peak.
 MLP_FLOPS_PER_PAIR = 2 * (9 * 64 + 64 * 64 + 64 * 3)
+# Neighbour build (no reference counterpart), compulsory bytes per particle:
+#   hash_count    pos.xyz 12 read; key, in-cell rank 8 written             = 20
+#   scatter       key, rank, cell start, index 16 read; 3 index arrays 12 written = 28
+#   rank_reorder  3 index arrays 12 + pos, vel 24 read; index 4 + pos, vel 24 written = 64
+# (the per-cell count/scan arrays are per cell, not per particle, and left out)
+BUILD_KERNELS = ("hash_count", "scan_reduce", "scan_top", "scan_down", "scatter", "rank_reorder")
+BUILD_BYTES = 20 + 28 + 64
 BF16_DENSE_TFLOPS = 2500.0
 
 
@@ -86,8 +93,22 @@ def cpu_baseline(pkg, scene, budget_s: float, state=None, state_step=0):
             break
     start = f"step {state_step + 1} of the run" if state_step else "the seeded lattice"
     return {"value": scene.count * nsteps / el, "unit": "particle-steps/s", "cores": threads, "kind": "port",
+            "host": host_cpu(),
             "sample": f"{scene.count} particles, {nsteps} steps of the FP64 grid transcription from {start} "
                       f"(after 1 untimed step), {threads} OpenMP threads, {el:.1f} s"}
+
+
+def host_cpu():
+    """CPU model and logical CPU count of this host (from /proc/cpuinfo)."""
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"model": model, "logical_cpus": os.cpu_count()}
 
 
 def progress(msg):
@@ -198,8 +219,9 @@ def learned_form(pkg, scenes, warmup, fine_steps, plain_ms_per_step, weights_pat
 def profile_record(workload, kernel, tuning):
     """The committed rocprofv3 record of `kernel` for this workload and tuning
     (profiles/traffic_<workload>.json), or None: HBM bytes per launch from
-    FETCH_SIZE / WRITE_SIZE calibrated by a plain copy, and the VALU-busy
-    fraction from SQ_ACTIVE_INST_VALU / SQ_BUSY_CYCLES."""
+    FETCH_SIZE / WRITE_SIZE calibrated by a plain copy, and the VALU issue
+    estimate (4 SIMD cycles per wave64 SQ_INSTS_VALU instruction over the
+    dispatch duration x 1024 SIMDs x 2.4 GHz)."""
     path = os.path.join(ROOT, "profiles", f"traffic_{workload}.json")
     if not os.path.exists(path):
         return None
@@ -287,10 +309,17 @@ def main(argv=None):
         roofline = {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                     "traffic_gbs": None if traffic is None else traffic / (dom_ms * 1e-3) / 1e9,
-                    "valu_busy": None if prof is None else prof.get("valu_busy"),
+                    "valu_issue_estimate": None if prof is None else prof.get("valu_issue_estimate"),
                     "algo_bytes_per_particle": algo, "launch_ms": dom_ms,
                     "binding_roof": "VALU issue: the neighbour gathers are instruction-bound (about 10 VALU "
                                     "instructions per candidate pair), with HBM mostly idle"}
+    build_ms = sum(kern[k]["ms_avg"] for k in BUILD_KERNELS if k in kern)
+    build = {"kernels": [k for k in BUILD_KERNELS if k in kern], "ms": build_ms,
+             "algo_bytes_per_particle": BUILD_BYTES,
+             "achieved": BUILD_BYTES * n / (build_ms * 1e-3) / 1e9 if build_ms > 0 else None,
+             "peak": HBM_PEAK_GBS, "unit": "GB/s"}
+    if build["achieved"] is not None:
+        build["frac"] = build["achieved"] / HBM_PEAK_GBS
     m = sim.metrics(0)
     sim.close()
 
@@ -323,6 +352,7 @@ def main(argv=None):
                    "tuning": tuning},
         "step_hbm_frac": STEP_BYTES * value / (HBM_PEAK_GBS * 1e9),
         "roofline": roofline,
+        "build_roofline": build,
         "kernels": kern,
         "cpu_baseline": cpu,
         "learned": learned,

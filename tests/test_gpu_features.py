@@ -280,3 +280,25 @@ def test_snapshot_save_load_resume(pkg, gpu_lib, tmp_path):
     with pkg.Simulation(sum(counts)) as c:  # one scene of the same total size
         with pytest.raises(pkg.SphError):
             c.load(path)
+
+
+def test_dense_cluster_long_rows(pkg, orc, gpu_lib):
+    """1400 particles within 0.2h next to a sparse cloud: stencil rows far longer
+    than one LDS stage round (128 candidates), waves that mix cluster and sparse
+    lanes.  Every pass matches the oracle and the fused step equals the passes."""
+    rng = np.random.default_rng(11)
+    n = 3200
+    p = orc.particles(n)
+    p["pos"][:, :3] = rng.uniform(-0.08, 0.08, (n, 3)).astype(np.float32)
+    p["pos"][:1400, :3] = (rng.uniform(0, 4e-3, (1400, 3)) + 0.01).astype(np.float32)
+    p["vel"][:, :3] = rng.normal(0, 0.3, (n, 3)).astype(np.float32)
+    ob = orc.default_boundary()
+    with pkg.Simulation(n) as sim, pkg.Simulation(n) as fused:
+        sim.upload(p)
+        s1, s2, s3 = _passes(sim)
+        fused.upload(p)
+        fused.step(1)
+        assert fused.download().tobytes() == s3.tobytes()
+    check_density(orc, p, s1, ob, grid=False)
+    check_force(orc, s1, s2, ob, grid=False)
+    check_integrate(orc, s2, s3, ob)

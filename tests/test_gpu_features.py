@@ -302,3 +302,36 @@ def test_dense_cluster_long_rows(pkg, orc, gpu_lib):
     check_density(orc, p, s1, ob, grid=False)
     check_force(orc, s1, s2, ob, grid=False)
     check_integrate(orc, s2, s3, ob)
+
+
+@pytest.mark.parametrize("sub", [1, 2, 3])
+def test_staged_rows_every_tuning(pkg, orc, gpu_lib, sub):
+    """A 128 x 8 x 16 slab along x in a widened box: almost every wave's
+    particles share one cell row, so the gathers take the LDS-staged path
+    (dam10k's short rows mostly take the global-memory one).  Every tuning
+    matches the oracle and gives the same bits."""
+    h = orc.smoothing_length()
+    sp = 0.5 * h
+    lower, upper = (-0.1, -0.35, -0.1), (1.4, 0.3, 0.3)
+    origin = (lower[0] + 0.5 * sp, lower[1] + 0.5 * sp, lower[2] + 0.5 * sp)
+    n = 128 * 8 * 16
+    p0 = orc.lattice(128, 8, 16, sp, origin, 0.05 * h, 7, count=n)
+    ob = orc.make_boundary(lower, upper)
+    with pkg.Simulation(n, boundaries=pkg.sph.make_boundary(lower, upper)) as sim:
+        sim.upload(p0)
+        sim.step(40)
+        st = sim.download()
+        results = []
+        for prune in (0, 1):
+            for block in (128, 256):
+                for bpx in (0, 2):
+                    sim.set_tuning(sub, prune, block, bpx)
+                    sim.upload(st)
+                    results.append(_passes(sim))
+    for r in results[1:]:
+        for x, y in zip(results[0], r):
+            assert x.tobytes() == y.tobytes()
+    s1, s2, s3 = results[0]
+    check_density(orc, st, s1, ob, grid=False)
+    check_force(orc, s1, s2, ob, grid=False)
+    check_integrate(orc, s2, s3, ob)

@@ -313,6 +313,19 @@ def main(argv=None):
                     "algo_bytes_per_particle": algo, "launch_ms": dom_ms,
                     "binding_roof": "VALU issue: the neighbour gathers are instruction-bound (about 10 VALU "
                                     "instructions per candidate pair), with HBM mostly idle"}
+    # Host readback of the whole AoS (sph_download into pageable numpy memory):
+    # what a frame would cost a caller that takes every step's particles to the
+    # host instead of binding the device buffer.  Untimed region; not `value`.
+    host = np.empty(n, dtype=pkg.sph.PARTICLE_DTYPE)
+    sim.download_into(host)
+    reps = 5
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        sim.download_into(host)
+    dl_ms = 1e3 * (time.perf_counter() - t0) / reps
+    pcie = {"download_ms_per_frame": dl_ms, "bytes": n * pkg.sph.PARTICLE_DTYPE.itemsize,
+            "gbs": n * pkg.sph.PARTICLE_DTYPE.itemsize / (dl_ms * 1e-3) / 1e9,
+            "value_with_download_every_step": n / ((ms_per_step + dl_ms) * 1e-3)}
     build_ms = sum(kern[k]["ms_avg"] for k in BUILD_KERNELS if k in kern)
     build = {"kernels": [k for k in BUILD_KERNELS if k in kern], "ms": build_ms,
              "algo_bytes_per_particle": BUILD_BYTES,
@@ -353,6 +366,7 @@ def main(argv=None):
         "step_hbm_frac": STEP_BYTES * value / (HBM_PEAK_GBS * 1e9),
         "roofline": roofline,
         "build_roofline": build,
+        "host_readback": pcie,
         "kernels": kern,
         "cpu_baseline": cpu,
         "learned": learned,

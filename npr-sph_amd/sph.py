@@ -323,6 +323,13 @@ class Simulation:
         self._check(self._L.sph_download(self._h, out.ctypes.data, first, n))
         return out
 
+    def download_into(self, out: np.ndarray, first: int = 0) -> np.ndarray:
+        """sph_download into a caller's contiguous PARTICLE_DTYPE array."""
+        if out.dtype != PARTICLE_DTYPE or not out.flags.c_contiguous:
+            raise ValueError("out must be a contiguous PARTICLE_DTYPE array")
+        self._check(self._L.sph_download(self._h, out.ctypes.data, first, len(out)))
+        return out
+
     def init_lattice(self, lat: Lattice, scene: int = 0):
         self._check(self._L.sph_init_lattice(self._h, scene, C.byref(lat)))
 

@@ -63,8 +63,10 @@ __device__ __forceinline__ void for_each_row(const SceneDev& sc, const uint32_t*
 
 // ------------------------------------------------------ LDS-staged walk ---
 
-// Candidates staged per wave per round.
-constexpr int kStage = 128;
+// Candidates staged per wave per round.  The stage's three planes (x, y, z)
+// of kStage floats then lie within 255 dwords of each other, the reach of a
+// ds_read2_b32 offset, so one address register serves all three.
+constexpr int kStage = 124;
 
 __device__ __forceinline__ int wave_min_i32(int v)
 {
@@ -113,8 +115,34 @@ __device__ __forceinline__ float4 f4_load(__amdgpu_buffer_rsrc_t r, uint32_t vof
 template <int SUB>
 constexpr int gather_lds_words()
 {
-    return 2 * (2 * SUB + 1) * 64 > 4 * kStage ? 2 * (2 * SUB + 1) * 64 : 4 * kStage;
+    return 2 * (2 * SUB + 1) * 64 > 3 * kStage ? 2 * (2 * SUB + 1) * 64 : 3 * kStage;
 }
+
+// Candidates in the LDS stage, stored as three planes x[kStage], y[kStage],
+// z[kStage].  Lanes reading different candidates then meet in one bank only
+// when the candidates are 32 apart (float4 records would collide every 8),
+// and two consecutive candidates are one ds_read2 per plane.  Indexing
+// yields (x, y, z, 0) and advancing moves along the candidates, as a
+// float4 pointer into the slot arrays does.
+struct StageRow {
+    const float* p;
+    __device__ __forceinline__ float4 operator[](uint32_t k) const
+    {
+        return make_float4(p[k], p[k + kStage], p[k + 2 * kStage], 0.f);
+    }
+    __device__ __forceinline__ float4 operator*() const { return (*this)[0]; }
+    __device__ __forceinline__ StageRow operator+(uint32_t k) const { return {p + k}; }
+    __device__ __forceinline__ StageRow& operator+=(uint32_t k)
+    {
+        p += k;
+        return *this;
+    }
+    __device__ __forceinline__ StageRow& operator++()
+    {
+        ++p;
+        return *this;
+    }
+};
 
 // Stencil walk of one wave.  When every valid lane of the wave lies in the
 // same cell row (cy, cz) - the common case, since slots are cell-sorted x
@@ -128,8 +156,11 @@ constexpr int gather_lds_words()
 // (rows in z, y order, slots ascending), so results do not depend on the way
 // taken.  Lanes with valid = false visit nothing but must take part: the call
 // is made by the whole wave.
-// rP views P (np elements).  body(q, m, j) visits the m candidates q[0..m)
-// whose first slot is j; q points into LDS or into P.
+// rP views P (np elements).  body(q, m, j, self_row) visits the m candidates
+// q[0..m) whose first slot is j; q is a StageRow in LDS or a float4 pointer
+// into P, and q[k] has the candidate's position in x, y, z.  self_row is
+// wave-uniform and false only where no lane's own slot can be among the
+// candidates: staged rows other than the wave's own cell row.
 template <int SUB, bool PRUNE, bool PREFETCH, class Body>
 __device__ __forceinline__ void gather_rows(const SceneDev& sc, const uint32_t* __restrict__ start,
                                             const float4* __restrict__ P, __amdgpu_buffer_rsrc_t rP, uint32_t* lds,
@@ -137,7 +168,7 @@ __device__ __forceinline__ void gather_rows(const SceneDev& sc, const uint32_t* 
                                             Body&& body)
 {
     constexpr int kRows = 2 * SUB + 1;
-    constexpr int kPer = kStage / 64;
+    constexpr int kPer = (kStage + 63) / 64;
     const int lane = (int)__lane_id();
     const int dx = sc.dims[0], dy = sc.dims[1], dz = sc.dims[2];
     const int rk = cz * dy + cy;
@@ -146,7 +177,7 @@ __device__ __forceinline__ void gather_rows(const SceneDev& sc, const uint32_t* 
     if (rmin != rmax) {
         if (valid)
             for_each_row<SUB, PRUNE, 64>(sc, start, fx, fy, fz, cx, cy, cz, lds + lane,
-                                         [&](uint32_t b, uint32_t e) { body(P + b, e > b ? e - b : 0u, b); });
+                                         [&](uint32_t b, uint32_t e) { body(P + b, e > b ? e - b : 0u, b, true); });
         return;
     }
     if (rmin == 0x7fffffff) return;  // no valid lane
@@ -155,7 +186,7 @@ __device__ __forceinline__ void gather_rows(const SceneDev& sc, const uint32_t* 
     const int ucy = row_yz % dy, ucz = row_yz / dy;
     const int xlo = __builtin_amdgcn_readfirstlane(max(wave_min_i32(valid ? cx : 0x7fffffff) - SUB, 0));
     const int xhi = __builtin_amdgcn_readfirstlane(min(wave_max_i32(valid ? cx : -1) + SUB, dx - 1));
-    float4* stage = reinterpret_cast<float4*>(lds);
+    float* stage = reinterpret_cast<float*>(lds);
     auto fetch = [&](float4 (&dst)[kPer], uint32_t c0, uint32_t n) {
 #pragma unroll
         for (int r = 0; r < kPer; ++r) {
@@ -167,7 +198,11 @@ __device__ __forceinline__ void gather_rows(const SceneDev& sc, const uint32_t* 
 #pragma unroll
         for (int r = 0; r < kPer; ++r) {
             const uint32_t i = (uint32_t)lane + 64u * (uint32_t)r;
-            if (i < n) stage[i] = src[r];
+            if (i < n) {
+                stage[i] = src[r].x;
+                stage[i + kStage] = src[r].y;
+                stage[i + 2 * kStage] = src[r].z;
+            }
         }
     };
     const int xa = cx > SUB ? cx - SUB : 0;
@@ -212,9 +247,11 @@ __device__ __forceinline__ void gather_rows(const SceneDev& sc, const uint32_t* 
             put(pre, n0);
             wave_lds_sync();
             if (PREFETCH && k + 1 < kRows) fetch(pre, RB[k + 1], min(RE[k + 1] - RB[k + 1], (uint32_t)kStage));
+            // Every valid lane's own slot lies in row (ucy, ucz): k == SUB of z == ucz.
+            const bool self_row = z == ucz && k == SUB;
             {
                 const uint32_t lo = max(rb[k], RB[k]), hi = min(re[k], RB[k] + n0);
-                if (hi > lo) body(stage + (lo - RB[k]), hi - lo, lo);
+                if (hi > lo) body(StageRow{stage + (lo - RB[k])}, hi - lo, lo, self_row);
             }
             wave_lds_sync();
             // Further rounds of a row longer than the stage.
@@ -225,7 +262,7 @@ __device__ __forceinline__ void gather_rows(const SceneDev& sc, const uint32_t* 
                 put(more, n);
                 wave_lds_sync();
                 const uint32_t lo = max(rb[k], c0), hi = min(re[k], c0 + n);
-                if (hi > lo) body(stage + (lo - c0), hi - lo, lo);
+                if (hi > lo) body(StageRow{stage + (lo - c0)}, hi - lo, lo, self_row);
                 wave_lds_sync();
             }
         }

@@ -465,7 +465,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(SUB < 3 ?
         const float h = sc.h, h2 = sc.h2;
         float ax = 0.f, ay = 0.f, az = 0.f;  // pressure sum
         float bx = 0.f, by = 0.f, bz = 0.f;  // viscosity sum
-        int head = 0, tail = 0;  // this lane's queue entries are [head, tail) mod kQueue
+        int head = 0, tail = 0;  // this lane's queue entries are [head, tail) mod kQueue; head is even
         // One neighbour's pressure and viscosity terms.  The hardware reciprocal
         // square root of r^2 = 0 (a coincident pair) is +inf, so r = 0*inf is
         // NaN, as normalize(0) in the reference.
@@ -486,7 +486,9 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(SUB < 3 ?
         };
         // Processes queue entries two per round (the loads of both are issued
         // before either is used), oldest first.  With all = false it returns
-        // with every lane at most kQueue - 8 entries full.
+        // with every lane at most kQueue - 8 entries full.  head stays even, so
+        // a round's two entries are adjacent slots (one ds_read2); a lane that
+        // had one entry moves its tail past the unused slot.
         auto flush = [&](bool all) {
             for (;;) {
                 const int n = tail - head;
@@ -494,13 +496,15 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(SUB < 3 ?
                         : (!__any(n > kQueue - 8) && __popcll(__ballot(n > 0)) < kDrainLanes))
                     break;
                 if (n > 0) {
-                    const uint32_t j0 = queue[head & (kQueue - 1)][threadIdx.x];
-                    const uint32_t j1 = n > 1 ? queue[(head + 1) & (kQueue - 1)][threadIdx.x] : sentinel;
+                    const uint32_t* e = &queue[head & (kQueue - 2)][threadIdx.x];
+                    const uint32_t j0 = e[0], e1 = e[BLOCK];
+                    const uint32_t j1 = n > 1 ? e1 : sentinel;
                     const float4 x0 = f4_load(rP, j0), v0 = f4_load(rV, j0);
                     const float4 x1 = f4_load(rP, j1), v1 = f4_load(rV, j1);
                     pair(x0, v0);
                     pair(x1, v1);
-                    head += n > 1 ? 2 : 1;
+                    head += 2;
+                    tail = max(tail, head);
                 }
             }
         };

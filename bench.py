@@ -311,8 +311,13 @@ def main(argv=None):
                     "traffic_gbs": None if traffic is None else traffic / (dom_ms * 1e-3) / 1e9,
                     "valu_issue_estimate": None if prof is None else prof.get("valu_issue_estimate"),
                     "algo_bytes_per_particle": algo, "launch_ms": dom_ms,
+                    # traffic and the VALU estimate come from the committed counter
+                    # runs, whose own launches took profile_launch_ms
+                    "profile": None if prof is None else prof.get("source"),
+                    "profile_launch_ms": None if prof is None else prof.get("rocprof_avg_ms_timed_launches"),
                     "binding_roof": "VALU issue: the neighbour gathers are instruction-bound (about 10 VALU "
-                                    "instructions per candidate pair), with HBM mostly idle"}
+                                    "instructions per candidate in the density scan, 13 in the force scan, "
+                                    "23 per neighbour in the force terms), with HBM mostly idle"}
     # Host readback of the whole AoS (sph_download into pageable numpy memory):
     # what a frame would cost a caller that takes every step's particles to the
     # host instead of binding the device buffer.  Untimed region; not `value`.

@@ -84,6 +84,33 @@ def test_batch_equals_unbatched(pkg, orc, gpu_lib):
         off += s.count
 
 
+def test_batch256_full_draw_equals_unbatched(pkg, gpu_lib):
+    """The whole 256-scene draw (about 9.5M particles) in one batch: the largest,
+    the smallest and two more of its scenes equal their own runs bit for bit."""
+    scenes = pkg.scenes.batch256()
+    assert len(scenes) == 256
+    sph = pkg.sph
+    bnds = [sph.make_boundary(s.lower, s.upper) for s in scenes]
+    counts = [s.count for s in scenes]
+    first = np.concatenate([[0], np.cumsum(counts)[:-1]])
+    picks = sorted({int(np.argmax(counts)), int(np.argmin(counts)), 17, 200})
+    with pkg.Simulation(counts, boundaries=bnds) as batch:
+        for i, s in enumerate(scenes):
+            batch.init_lattice(sph.make_lattice(**s.lattice_args()), scene=i)
+        batch.step(40)
+        got = batch.download()
+        metrics = {i: batch.metrics(i) for i in picks}
+    for i in picks:
+        s = scenes[i]
+        with pkg.Simulation(s.count, boundaries=bnds[i]) as one:
+            one.init_lattice(sph.make_lattice(**s.lattice_args()))
+            one.step(40)
+            alone = one.download()
+            m = one.metrics(0)
+        assert got[first[i]:first[i] + s.count].tobytes() == alone.tobytes(), f"scene {i} differs"
+        assert metrics[i].mean_rho == m.mean_rho
+
+
 def test_batch_scene_parity_vs_oracle(pkg, orc, gpu_lib):
     scenes = [s for s in pkg.scenes.batch256() if s.count < 8000][:3]
     sph = pkg.sph

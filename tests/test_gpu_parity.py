@@ -199,6 +199,32 @@ def test_dam1m_one_step_vs_grid_oracle(pkg, orc, gpu_lib):
         assert_same_grid(start, perm, cstart, cperm, [g.ncells])
 
 
+def test_dam8m_one_step_vs_grid_oracle(pkg, orc, gpu_lib):
+    """8M particles (the large mixed run's scene) during its collapse: per-pass
+    parity against the FP64 grid transcription and the bit-exact grid.  Its
+    5 GB working set exceeds the Infinity Cache, unlike the 1M state."""
+    sc = pkg.scenes.dam8m()
+    b = pkg.sph.make_boundary(sc.lower, sc.upper)
+    ob = orc.make_boundary(sc.lower, sc.upper)
+    with pkg.Simulation(sc.count, boundaries=b) as sim:
+        sim.init_lattice(pkg.sph.make_lattice(**sc.lattice_args()))
+        sim.step(150)
+        st = sim.download()
+        sim.rho_pres()
+        s1 = sim.download()
+        check_density(orc, st, s1, ob, grid=True)
+        sim.force()
+        s2 = sim.download()
+        check_force(orc, s1, s2, ob, grid=True)
+        sim.integrate()
+        s3 = sim.download()
+        check_integrate(orc, s2, s3, ob)
+        start, perm = sim.download_grid()
+        g = orc.grid_geometry(orc.smoothing_length(), ob, sc.count, sim.grid_info(0).cells_per_h)
+        _, cstart, cperm = orc.grid_build(s2["pos"][:, :3], g)
+        assert_same_grid(start, perm, cstart, cperm, [g.ncells])
+
+
 def test_reference_make_grid_scene(pkg, orc, gpu_lib):
     """The reference's literal initial state (make_grid, Main.cpp:488-505: 10x100x10
     at spacing PARTICLE_RADIUS = h/4, ~251 neighbours) through all three passes,

@@ -40,6 +40,9 @@ HBM_PEAK_GBS = 8000.0
 #                    force, pos, vel 36 written                             = 68
 ALGO_BYTES = {"density": 20, "force": 44, "integrate": 64, "force_integrate": 68}
 STEP_BYTES = 20 + 44 + 64  # the three reference passes, 128 B per particle-step
+# The streaming integrate kernel over the AoS buffer moves whole float4s: the
+# 64-byte record read, pos and vel (32 bytes) written.
+INTEGRATE_MOVED_BYTES = 64 + 32
 # Learned message pass: flops of the pair MLP 9 -> 64 -> 64 -> 3 per neighbour
 # pair (multiply-adds counted as 2), against MI355X's dense BF16 MFMA peak.
 MLP_FLOPS_PER_PAIR = 2 * (9 * 64 + 64 * 64 + 64 * 3)
@@ -231,6 +234,43 @@ def profile_record(workload, kernel, tuning):
     return rec
 
 
+def three_passes(sim, n, frames: int = 5):
+    """The reference's operator surface, one call per pass as Main.cpp:291-304
+    dispatches them (sph_rho_pres rebuilds the grid, then sph_force and
+    sph_integrate), after the timed region.  Reports the wall time per frame of
+    the ungraphed calls and, from per-pass HIP events, the streaming integrate
+    kernel against HBM: 64 algorithmic bytes per particle, 96 moved (it reads
+    whole 64-byte records to keep their w lanes and writes pos and vel)."""
+    sim.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(frames):
+        sim.rho_pres()
+        sim.force()
+        sim.integrate()
+    sim.synchronize()
+    frame_ms = 1e3 * (time.perf_counter() - t0) / frames
+    sim.set_timing(True)
+    for _ in range(frames):
+        sim.rho_pres()
+        sim.force()
+        sim.integrate()
+    sim.synchronize()
+    times = sim.kernel_times()
+    sim.set_timing(False)
+    kern = {k: {"ms_avg": v[0] / v[1], "launches": v[1]} for k, v in times.items()}
+    out = {"frames": frames, "ms_per_frame": frame_ms, "kernels": kern}
+    if "integrate" in kern:
+        ms = kern["integrate"]["ms_avg"]
+        out["integrate_roofline"] = {"bound": "hbm", "kernel": "integrate", "launch_ms": ms,
+                                     "algo_bytes_per_particle": ALGO_BYTES["integrate"],
+                                     "moved_bytes_per_particle": INTEGRATE_MOVED_BYTES,
+                                     "achieved": ALGO_BYTES["integrate"] * n / (ms * 1e-3) / 1e9,
+                                     "moved_gbs": INTEGRATE_MOVED_BYTES * n / (ms * 1e-3) / 1e9,
+                                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                     "frac": ALGO_BYTES["integrate"] * n / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
+    return out
+
+
 def copy_probe(mbytes: int, reps: int = 20):
     """Plain device copy of known size (calibrates rocprof FETCH/WRITE counters)."""
     import torch
@@ -339,6 +379,7 @@ def main(argv=None):
     if build["achieved"] is not None:
         build["frac"] = build["achieved"] / HBM_PEAK_GBS
     m = sim.metrics(0)
+    passes = three_passes(sim, n)
     sim.close()
 
     progress(f"plain: {ms_per_step:.3f} ms/step")
@@ -371,6 +412,7 @@ def main(argv=None):
         "step_hbm_frac": STEP_BYTES * value / (HBM_PEAK_GBS * 1e9),
         "roofline": roofline,
         "build_roofline": build,
+        "three_passes": passes,
         "host_readback": pcie,
         "kernels": kern,
         "cpu_baseline": cpu,

@@ -581,21 +581,49 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(SUB < 3 ?
     });
 }
 
-// integrate_comp.glsl:35-81 as a streaming pass over the AoS buffer.
+// Value of lane K of this lane's quad (DPP quad permute); every lane of the
+// quad must be active.
+template <int K>
+__device__ __forceinline__ float quad_lane(float v)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), K * 0x55, 0xF, 0xF, false));
+}
+
+// integrate_comp.glsl:35-81 as a streaming pass over the AoS buffer.  Lane q of
+// a quad reads float4 q of one 64-byte record (pos, vel, force, extras), so a
+// wave's loads are one contiguous 1 KB run.  The quad shares the record
+// through DPP, every lane applies integrate_one, and lanes 0 and 1 write pos
+// and vel back with their w lanes unchanged.
 __global__ __launch_bounds__(kTile) void k_integrate_aos(const Tile* __restrict__ tiles, const SceneDev* __restrict__ scenes,
                                                          sph_particle_t* __restrict__ aos)
 {
+    static_assert(kTile % 4 == 0, "four lanes per record");
     const Tile t = tiles[blockIdx.x];
     const SceneDev& sc = scenes[t.scene];
-    const uint32_t i = t.begin + threadIdx.x;
-    if (i >= t.end) return;
-    float4* r = reinterpret_cast<float4*>(aos + i);
-    const float4 p = r[0], v = r[1], f4 = r[2], e = r[3];
-    const float x[3] = {p.x, p.y, p.z}, vel[3] = {v.x, v.y, v.z}, f[3] = {f4.x, f4.y, f4.z};
-    float nx[3], nv[3];
-    integrate_one(sc, x, vel, f, e.x, nx, nv);
-    r[0] = make_float4(nx[0], nx[1], nx[2], p.w);
-    r[1] = make_float4(nv[0], nv[1], nv[2], v.w);
+    float4* r4 = reinterpret_cast<float4*>(aos);
+    const uint32_t q = threadIdx.x & 3u;
+    // All four rounds' loads are issued before the first store; quads past
+    // the tile's end read its first record instead and store nothing.
+    float4 rec[4];
+#pragma unroll
+    for (uint32_t k = 0; k < 4u; ++k) {
+        const uint32_t i = t.begin + k * (uint32_t)(kTile / 4) + threadIdx.x / 4u;
+        rec[k] = r4[4 * (size_t)(i < t.end ? i : t.begin) + q];
+    }
+#pragma unroll
+    for (uint32_t k = 0; k < 4u; ++k) {
+        const uint32_t i = t.begin + k * (uint32_t)(kTile / 4) + threadIdx.x / 4u;
+        const bool live = i < t.end;  // uniform across the quad
+        const float4 v = rec[k];
+        const float x[3] = {quad_lane<0>(v.x), quad_lane<0>(v.y), quad_lane<0>(v.z)};
+        const float vel[3] = {quad_lane<1>(v.x), quad_lane<1>(v.y), quad_lane<1>(v.z)};
+        const float f[3] = {quad_lane<2>(v.x), quad_lane<2>(v.y), quad_lane<2>(v.z)};
+        const float rho = quad_lane<3>(v.x);
+        float nx[3], nv[3];
+        integrate_one(sc, x, vel, f, rho, nx, nv);
+        if (live && q < 2u)
+            r4[4 * (size_t)i + q] = q == 0u ? make_float4(nx[0], nx[1], nx[2], v.w) : make_float4(nv[0], nv[1], nv[2], v.w);
+    }
 }
 
 // ------------------------------------------------------------- reductions ---

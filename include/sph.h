@@ -112,7 +112,7 @@ typedef struct sph_grid_info {
  * gives results within the FP32 tolerance of the FP64 transcription; a fixed
  * setting gives bit-identical results run to run, and prune/block/
  * blocks_per_xcd never change the bits (only cells_per_h changes the
- * neighbour summation order).  Defaults: {2, 1, 256, 0}. */
+ * neighbour summation order).  Defaults: {2, 1, 128, 0}. */
 typedef struct sph_tuning {
     int32_t cells_per_h;     /* 1..3: cell edge = h*(1+2^-10)/cells_per_h */
     int32_t prune;           /* 1: skip stencil rows and cells farther than h from the particle */
@@ -226,8 +226,10 @@ int sph_metrics(sph_ctx* ctx, int scene, sph_metrics_t* out);
 
 /* Gather tuning (no reference counterpart).  sph_autotune times every
  * supported setting for `trial_steps` steps from the current state, re-times
- * the four fastest for 4*trial_steps steps, restores that state, applies the
- * fastest setting and reports it with its per-step milliseconds. */
+ * the four fastest and the setting in force for 4*trial_steps steps, restores
+ * that state, and applies the fastest of those if it beats the setting in
+ * force by more than 2% (otherwise keeps the setting in force).  It reports
+ * the applied setting with its per-step milliseconds. */
 int sph_get_tuning(const sph_ctx* ctx, sph_tuning_t* out);
 int sph_set_tuning(sph_ctx* ctx, const sph_tuning_t* t);
 int sph_autotune(sph_ctx* ctx, int trial_steps, sph_tuning_t* best, double* best_ms_per_step);

@@ -210,7 +210,7 @@ void destroy_graphs(sph_ctx* c)
     }
 }
 
-sph_tuning_t default_tuning() { return sph_tuning_t{2, 1, 256, 0}; }
+sph_tuning_t default_tuning() { return sph_tuning_t{2, 1, 128, 0}; }
 
 void select_gather_tiles(sph_ctx* c)
 {
@@ -1066,7 +1066,16 @@ int sph_autotune(sph_ctx* c, int trial_steps, sph_tuning_t* best, double* best_m
         return SPH_OK;
     };
     // Screen every setting with trial_steps steps, then re-time the four
-    // fastest with four times as many and keep the fastest of those.
+    // fastest and the setting in force at the call with four times as many.
+    // The fastest finalist replaces the setting in force only when it is more
+    // than kTieMargin faster, so near-ties (timing noise between settings
+    // whose step times differ by a percent) keep the caller's setting and
+    // repeated calls agree.
+    constexpr float kTieMargin = 0.02f;
+    auto same = [](const sph_tuning_t& a, const sph_tuning_t& b) {
+        return a.cells_per_h == b.cells_per_h && a.prune == b.prune && a.block == b.block &&
+               a.blocks_per_xcd == b.blocks_per_xcd;
+    };
     std::vector<std::pair<float, sph_tuning_t>> screened;
     const int32_t bpx_options[2] = {0, 64};
     for (int32_t sub = 1; sub <= 3 && rc == SPH_OK; ++sub)
@@ -1080,15 +1089,25 @@ int sph_autotune(sph_ctx* c, int trial_steps, sph_tuning_t* best, double* best_m
                 }
     std::stable_sort(screened.begin(), screened.end(),
                      [](const auto& a, const auto& b) { return a.first < b.first; });
+    std::vector<sph_tuning_t> finalists;
+    for (size_t k = 0; k < screened.size() && k < 4; ++k) finalists.push_back(screened[k].second);
+    bool start_in = false;
+    for (const auto& t : finalists) start_in = start_in || same(t, start_tuning);
+    if (!start_in) finalists.push_back(start_tuning);
     sph_tuning_t win = start_tuning;
-    float win_ms = 1e30f;
-    for (size_t k = 0; k < screened.size() && k < 4 && rc == SPH_OK; ++k) {
+    float win_ms = 1e30f, start_ms = 1e30f;
+    for (size_t k = 0; k < finalists.size() && rc == SPH_OK; ++k) {
         float ms = 0.f;
-        if ((rc = trial(screened[k].second, 4 * trial_steps, &ms)) != SPH_OK) break;
+        if ((rc = trial(finalists[k], 4 * trial_steps, &ms)) != SPH_OK) break;
+        if (same(finalists[k], start_tuning)) start_ms = ms;
         if (ms < win_ms) {
             win_ms = ms;
-            win = screened[k].second;
+            win = finalists[k];
         }
+    }
+    if (rc == SPH_OK && !same(win, start_tuning) && win_ms > start_ms * (1.f - kTieMargin)) {
+        win = start_tuning;
+        win_ms = start_ms;
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);

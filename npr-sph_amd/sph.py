@@ -393,7 +393,7 @@ class Simulation:
         self._check(self._L.sph_get_tuning(self._h, C.byref(t)))
         return t
 
-    def set_tuning(self, cells_per_h=2, prune=1, block=256, blocks_per_xcd=0):
+    def set_tuning(self, cells_per_h=2, prune=1, block=128, blocks_per_xcd=0):
         t = Tuning(cells_per_h, prune, block, blocks_per_xcd)
         self._check(self._L.sph_set_tuning(self._h, C.byref(t)))
 

@@ -7,8 +7,11 @@ spacing 0.5h with the reference's constants, initialised on the device.
 
 Printed JSON line (one):
   value        particle-steps/s over the timed steps (inputs resident in HBM)
+  repeats      median, min and max ms/step of 5 more blocks of steps
   roofline     the dominant kernel: algorithmic bytes per launch / its mean
                duration (HIP events on the solver's stream) vs 8 TB/s
+  three_passes the reference's ungraphed three-call surface per frame and the
+               streaming integrate kernel's HBM rate
   cpu_baseline the FP64 oracle (grid form, OpenMP) on this host, on a bounded
                sample of the same workload: steps from the GPU's warmed state
                until the CPU budget is spent
@@ -43,6 +46,9 @@ STEP_BYTES = 20 + 44 + 64  # the three reference passes, 128 B per particle-step
 # The streaming integrate kernel over the AoS buffer moves whole float4s: the
 # 64-byte record read, pos and vel (32 bytes) written.
 INTEGRATE_MOVED_BYTES = 64 + 32
+# Repeated blocks of steps after the timed region (spread of the step time).
+REPEAT_BLOCKS = 5
+REPEAT_BLOCK_STEPS = 200
 # Learned message pass: flops of the pair MLP 9 -> 64 -> 64 -> 3 per neighbour
 # pair (multiply-adds counted as 2), against MI355X's dense BF16 MFMA peak.
 MLP_FLOPS_PER_PAIR = 2 * (9 * 64 + 64 * 64 + 64 * 3)
@@ -332,6 +338,18 @@ def main(argv=None):
     ms_per_step = 1e3 * elapsed / args.steps
     value = n * args.steps / elapsed
 
+    # Spread: REPEAT_BLOCKS more blocks of graph-launched steps after the timed
+    # region, each bracketed by synchronisation (not part of `value`).
+    block_steps = max(1, min(REPEAT_BLOCK_STEPS, args.steps))
+    blocks = []
+    for _ in range(REPEAT_BLOCKS):
+        t1 = time.perf_counter()
+        sim.step(block_steps)
+        sim.synchronize()
+        blocks.append(1e3 * (time.perf_counter() - t1) / block_steps)
+    repeats = {"blocks": REPEAT_BLOCKS, "steps_per_block": block_steps, "median_ms_per_step": float(np.median(blocks)),
+               "min_ms_per_step": min(blocks), "max_ms_per_step": max(blocks)}
+
     # Per-kernel durations, HIP events on the solver's stream (ungraphed launches).
     sim.set_timing(True)
     sim.step(args.timing_steps)
@@ -401,6 +419,7 @@ def main(argv=None):
         "steps": args.steps,
         "warmup": args.warmup,
         "ms_per_step": ms_per_step,
+        "repeats": repeats,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,

@@ -51,3 +51,19 @@ def test_header_compiles_as_c11_and_links(tmp_path, pkg):
     assert cc.returncode == 0, cc.stderr
     run = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
     assert run.returncode == 0, (run.returncode, run.stderr)
+
+
+@pytest.mark.skipif(shutil.which("g++") is None, reason="needs a C++ compiler")
+def test_header_links_from_cpp(tmp_path, pkg):
+    """The extern "C" guard: a C++ caller (as Main.cpp would be) links unmangled names."""
+    lib = pkg.sph.library_path()
+    src = tmp_path / "use_sph.cpp"
+    src.write_text(PROGRAM.replace("_Static_assert", "static_assert"))
+    exe = tmp_path / "use_sph_cpp"
+    libdir = os.path.dirname(lib)
+    cc = subprocess.run(["g++", "-std=c++17", "-Wall", "-Wextra", "-Werror", "-I", os.path.join(ROOT, "include"),
+                         str(src), "-L", libdir, "-lnprsph", f"-Wl,-rpath,{libdir}", "-o", str(exe)],
+                        capture_output=True, text=True)
+    assert cc.returncode == 0, cc.stderr
+    run = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
+    assert run.returncode == 0, (run.returncode, run.stderr)

@@ -202,8 +202,14 @@ int validate_params(sph_ctx* c, const sph_constants_t& k, const sph_boundary_t& 
     return SPH_OK;
 }
 
+// Graph executables may still be running on the stream (sph_step returns
+// after enqueueing them, and a parameter change that moves the grid destroys
+// them), so the stream is drained before they are destroyed.
 void destroy_graphs(sph_ctx* c)
 {
+    bool any = false;
+    for (auto g : c->graph) any = any || g != nullptr;
+    if (any) (void)hipStreamSynchronize(c->stream);
     for (auto& g : c->graph) {
         if (g) (void)hipGraphExecDestroy(g);
         g = nullptr;

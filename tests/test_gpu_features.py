@@ -214,6 +214,27 @@ def test_force_on_uploaded_extras(pkg, orc, gpu_lib):
     check_force(orc, s1, s2, ob, grid=False)
 
 
+def test_regrid_right_after_enqueued_steps(pkg, orc, gpu_lib):
+    """A parameter change that moves the grid replaces the captured step graphs;
+    issued right after sph_step returns (graphs possibly still running), it
+    gives the same bits as issuing it after a synchronisation."""
+    sc = pkg.scenes.dam10k()
+    p0 = _lattice(orc, sc)
+    b = pkg.sph.make_boundary(sc.lower, sc.upper)
+    c7 = pkg.sph.Constants(0.02, 7.0, 3000.0, 1000.0)
+    with pkg.Simulation(sc.count, boundaries=b) as eager, pkg.Simulation(sc.count, boundaries=b) as synced:
+        for sim in (eager, synced):
+            sim.upload(p0)
+        eager.step(48)
+        eager.set_constants(c7)
+        eager.step(20)
+        synced.step(48)
+        synced.synchronize()
+        synced.set_constants(c7)
+        synced.step(20)
+        assert eager.download().tobytes() == synced.download().tobytes()
+
+
 def test_set_constants_regrids(pkg, orc, gpu_lib):
     """The reference's smoothing slider (Main.cpp:244) changes h; the grid follows."""
     sc = pkg.scenes.dam10k()

@@ -49,6 +49,9 @@ INTEGRATE_MOVED_BYTES = 64 + 32
 # Repeated blocks of steps after the timed region (spread of the step time).
 REPEAT_BLOCKS = 5
 REPEAT_BLOCK_STEPS = 200
+# Coarse steps of the learned form timed from the plain run's warmed state,
+# beside the plain steps they stand in for.
+LEARNED_WINDOW = 10
 # Learned message pass: flops of the pair MLP 9 -> 64 -> 64 -> 3 per neighbour
 # pair (multiply-adds counted as 2), against MI355X's dense BF16 MFMA peak.
 MLP_FLOPS_PER_PAIR = 2 * (9 * 64 + 64 * 64 + 64 * 3)
@@ -130,7 +133,9 @@ def density_timeline(sim, total_fine, every_fine, coarse=None):
     Also returns the final kinetic energy and centre of mass."""
     out = []
     m = None
-    for _ in range(total_fine // every_fine):
+    for k in range(total_fine // every_fine):
+        if k and k % 50 == 0:
+            progress(f"  timeline: {k * every_fine} of {total_fine} fine steps")
         if coarse is None:
             sim.step(every_fine)
         else:
@@ -178,18 +183,42 @@ def learned_form(pkg, scenes, warmup, fine_steps, plain_ms_per_step, weights_pat
         mfma = {"bound": "mfma", "kernel": "mlp_messages", "achieved": tflops, "peak": BF16_DENSE_TFLOPS,
                 "unit": "TFLOP/s", "frac": tflops / BF16_DENSE_TFLOPS, "launch_ms": msg_ms,
                 "pairs_per_launch": int(mst.pairs), "flops_per_pair": MLP_FLOPS_PER_PAIR,
-                "binding_roof": "VALU issue: 128 GELU evaluations per pair run on the vector ALUs between the "
-                                "MFMA layers"}
+                "ps_per_pair": msg_ms * 1e9 / mst.pairs,
+                "binding_roof": "no single unit: the vector ALUs issue about half the time (the GELU is a table "
+                                "lookup in LDS), the matrix cores an eighth, and the rest waits on table reads, "
+                                "neighbour loads and each layer's MFMA results"}
+    # The same window from the same state for both forms: LEARNED_WINDOW coarse
+    # steps and the 4x as many plain steps they stand in for.  The whole-run
+    # figure above also carries the learned rollout's drift (its pair count
+    # grows as it compresses the fluid further than plain SPH does).
+    sim.upload(s0)
+    sim.synchronize()
+    t1 = time.perf_counter()
+    sim.coarse_step(LEARNED_WINDOW, **judge)
+    sim.synchronize()
+    win_coarse_ms = 1e3 * (time.perf_counter() - t1) / LEARNED_WINDOW
+    sim.upload(s0)
+    sim.step(1)
+    sim.synchronize()
+    t1 = time.perf_counter()
+    sim.step(4 * LEARNED_WINDOW)
+    sim.synchronize()
+    win_plain_ms = 1e3 * (time.perf_counter() - t1) / (4 * LEARNED_WINDOW)
+    window = {"coarse_steps": LEARNED_WINDOW, "ms_per_coarse_step": win_coarse_ms, "plain_ms_per_step": win_plain_ms,
+              "speedup_vs_plain": 4.0 * win_plain_ms / win_coarse_ms}
     progress(f"learned: {ncoarse} coarse steps in {el:.2f} s; accuracy timelines")
     # accuracy: mean density every `every` fine steps of simulated time
     every = 20 if n <= 2_000_000 else 100
     sim.upload(s0)
     plain, ke_p, com_p = density_timeline(sim, fine_steps, every)
+    progress("learned: plain timeline done")
     sim.upload(s0)
     learned, ke_l, com_l = density_timeline(sim, fine_steps, every, coarse=judge)
+    progress("learned: learned timeline done")
     sim.upload(s0)
     ablation, ke_a, com_a = density_timeline(sim, fine_steps, every,
                                              coarse=dict(use_network=False, judge_every=0, drift_limit=0.0, cooldown=0))
+    progress("learned: no-network timeline done")
     sim.close()
 
     def err(x):
@@ -210,6 +239,7 @@ def learned_form(pkg, scenes, warmup, fine_steps, plain_ms_per_step, weights_pat
         "plain_sim_seconds_per_wall_second": 1e-4 / (plain_ms_per_step * 1e-3),
         "speedup_vs_plain": plain_ms_per_4 / ms_per_coarse,
         "coarse_steps": ncoarse,
+        "same_state_window": window,
         "mfma_roofline": mfma,
         "judge": st.as_dict(),
         "drift_definition": "judge every 50 coarse steps: |mean rho(1 coarse step) - mean rho(4 plain steps)| from the "

@@ -184,9 +184,8 @@ def learned_form(pkg, scenes, warmup, fine_steps, plain_ms_per_step, weights_pat
                 "unit": "TFLOP/s", "frac": tflops / BF16_DENSE_TFLOPS, "launch_ms": msg_ms,
                 "pairs_per_launch": int(mst.pairs), "flops_per_pair": MLP_FLOPS_PER_PAIR,
                 "ps_per_pair": msg_ms * 1e9 / mst.pairs,
-                "binding_roof": "no single unit: the vector ALUs issue about half the time (the GELU is a table "
-                                "lookup in LDS), the matrix cores an eighth, and the rest waits on table reads, "
-                                "neighbour loads and each layer's MFMA results"}
+                "binding_roof": "VALU issue: 128 GELU evaluations per pair run on the vector ALUs between the "
+                                "MFMA layers"}
     # The same window from the same state for both forms: LEARNED_WINDOW coarse
     # steps and the 4x as many plain steps they stand in for.  The whole-run
     # figure above also carries the learned rollout's drift (its pair count

@@ -97,11 +97,9 @@ struct MlpWork {
     __bf16* pack_a = nullptr;  // MFMA operand images (k_mlp_pack)
     float* pack_c = nullptr;
     unsigned long long* pairs = nullptr;  // pairs evaluated by the last MFMA message pass
-    uint16_t* gelu_tab = nullptr;         // kGeluTab BF16 GELU values by FP16 input bucket (k_gelu_table)
     bool pack_stale = true;    // params changed since the last pack
 };
 
-constexpr int kGeluTab = 8192;  // FP16 bit patterns >> 3
 constexpr int kMlpPackA = 4 * 64 * 8 + 4 * 2 * 64 * 8 + 2 * 64 * 8;  // bf16 entries
 constexpr int kMlpPackC = 4 * 64 * 4 * 2 + 64 * 4;                  // float entries
 

@@ -505,46 +505,24 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef _Float16 f16x2 __attribute__((ext_vector_type(2)));
 
-// GELU by table.  An FP16 input's top 13 bits (sign, exponent and 7 mantissa
-// bits) select one of kGeluTab buckets of 8 consecutive bit patterns; entry i
-// holds the BF16 rounding of gelu_poly at the bucket's midpoint, written with
-// c = clamp(x, -3, 3) as
+// gelu_poly of two values in packed FP16 arithmetic, rounded to BF16 for the
+// next MFMA.  Written with c = clamp(x, -3, 3):
 //   0.5c + q(c^2) + max(x - c, 0),
 // which equals gelu_poly for every finite x (q(9) - 1.5 is the same offset
-// beyond the knee in both) and stays finite for x = -inf.  The bucket is 2^-7
-// of the input wide relative to its value, so an entry is within about one
-// BF16 step of gelu_poly at any input of the bucket (the slope of gelu_poly is
-// at most 1.13): BF16 accuracy over the whole FP16 range with no tail cases.
-// The infinity buckets hold gelu_poly of +-inf themselves, NaN buckets NaN.
-__global__ __launch_bounds__(256) void k_gelu_table(uint16_t* __restrict__ tab)
-{
-    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
-    if (i >= (uint32_t)kGeluTab) return;
-    const bool top_exponent = ((i >> 7) & 0x1Fu) == 0x1Fu;  // infinity or NaN patterns
-    const uint16_t pattern = (uint16_t)(top_exponent ? (i << 3) : ((i << 3) | 4u));
-    const float x = (float)__builtin_bit_cast(_Float16, pattern);
-    float g;
-    if (x != x) {
-        g = x;
-    } else {
-        const float c = fminf(fmaxf(x, -kGeluKnee), kGeluKnee);
-        const float sq = c * c;
-        const float q = fmaf(fmaf(fmaf(kGeluC3, sq, kGeluC2), sq, kGeluC1), sq, kGeluC0) * sq;
-        g = fmaf(0.5f, c, q) + fmaxf(x - c, 0.f);
-    }
-    tab[i] = __builtin_bit_cast(uint16_t, (__bf16)g);
-}
-
-// GELU of two values through the table in LDS: one packed conversion to FP16
-// (rounding toward zero: the input then lies at most one FP16 step below its
-// bucket, whose width is eight steps) and two 16-bit reads, the results packed
-// as the next MFMA's BF16 operand.
+// beyond the knee in both) and stays finite for x = -inf.  FP16 carries 11
+// significant bits against BF16's 8, so the result is the BF16 rounding of
+// gelu_poly to within one BF16 step.
 typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
-__device__ __forceinline__ bf16x2 gelu2_tab(const uint16_t* tab, float a, float b)
+__device__ __forceinline__ bf16x2 gelu2_bf16(float a, float b)
 {
-    const uint32_t bits = __builtin_bit_cast(uint32_t, __builtin_amdgcn_cvt_pkrtz(a, b));
-    const uint32_t lo = tab[(bits & 0xFFFFu) >> 3], hi = tab[bits >> 19];
-    return __builtin_bit_cast(bf16x2, lo | (hi << 16));
+    const f16x2 x = {(_Float16)a, (_Float16)b};
+    const f16x2 knee = (_Float16)kGeluKnee;
+    const f16x2 c = __builtin_elementwise_min(__builtin_elementwise_max(x, -knee), knee);
+    const f16x2 sq = c * c;
+    const f16x2 q = ((((f16x2)(_Float16)kGeluC3 * sq + (_Float16)kGeluC2) * sq + (_Float16)kGeluC1) * sq +
+                     (_Float16)kGeluC0) * sq;
+    const f16x2 g = (f16x2)(_Float16)0.5f * c + q + __builtin_elementwise_max(x - c, (f16x2)(_Float16)0.f);
+    return bf16x2{(__bf16)(float)g.x, (__bf16)(float)g.y};
 }
 
 __host__ __device__ __forceinline__ int kslot_unit(int kslot)
@@ -598,14 +576,9 @@ __global__ __launch_bounds__(256) void k_coarse_mlp_mfma(const SceneDev* __restr
                                                          const float* __restrict__ rho, const float2* __restrict__ D2,
                                                          const __bf16* __restrict__ pa, const float* __restrict__ pc,
                                                          MlpNorm nm, uint32_t n, float* __restrict__ pred,
-                                                         unsigned long long* __restrict__ pair_count,
-                                                         const uint16_t* __restrict__ gelu_tab)
+                                                         unsigned long long* __restrict__ pair_count)
 {
     __shared__ uint32_t hits_all[4][kHitCap];
-    __shared__ __align__(16) uint16_t gtab[kGeluTab];
-    for (int k = (int)threadIdx.x; k < kGeluTab / 8; k += 256)
-        reinterpret_cast<uint4*>(gtab)[k] = reinterpret_cast<const uint4*>(gelu_tab)[k];
-    __syncthreads();
     uint32_t wave_pairs = 0;  // pairs this wave evaluated, added to *pair_count at the end
     const int lane = (int)(threadIdx.x & 63);
     const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
@@ -679,7 +652,7 @@ __global__ __launch_bounds__(256) void k_coarse_mlp_mfma(const SceneDev* __restr
                     for (int mm = 0; mm < 2; ++mm)
 #pragma unroll
                         for (int e = 0; e < 4; e += 2) {
-                            const bf16x2 g = gelu2_tab(gtab, h1[2 * ks + mm][e], h1[2 * ks + mm][e + 1]);
+                            const bf16x2 g = gelu2_bf16(h1[2 * ks + mm][e], h1[2 * ks + mm][e + 1]);
                             b2[ks][4 * mm + e] = g.x;
                             b2[ks][4 * mm + e + 1] = g.y;
                         }
@@ -696,7 +669,7 @@ __global__ __launch_bounds__(256) void k_coarse_mlp_mfma(const SceneDev* __restr
                     for (int mm = 0; mm < 2; ++mm)
 #pragma unroll
                         for (int e = 0; e < 4; e += 2) {
-                            const bf16x2 g = gelu2_tab(gtab, h2[2 * ks + mm][e], h2[2 * ks + mm][e + 1]);
+                            const bf16x2 g = gelu2_bf16(h2[2 * ks + mm][e], h2[2 * ks + mm][e + 1]);
                             b3[ks][4 * mm + e] = g.x;
                             b3[ks][4 * mm + e + 1] = g.y;
                         }
@@ -870,7 +843,6 @@ hipError_t launch_coarse_integrate(hipStream_t st, const DeviceState& d, const M
 hipError_t launch_mlp_pack(hipStream_t st, const MlpWork& w)
 {
     hipLaunchKernelGGL(k_mlp_pack, dim3(1), dim3(256), 0, st, w.params, w.pack_a, w.pack_c);
-    hipLaunchKernelGGL(k_gelu_table, dim3(kGeluTab / 256), dim3(256), 0, st, w.gelu_tab);
     return hipGetLastError();
 }
 
@@ -883,7 +855,7 @@ hipError_t launch_coarse_mlp(hipStream_t st, const DeviceState& d, int nscenes, 
         const dim3 grid(std::min<uint32_t>((d.n + 3u) / 4u, 256u * 64u)), block(256);
 #define NPRSPH_MFMA(S)                                                                                             \
     hipLaunchKernelGGL(k_coarse_mlp_mfma<S>, grid, block, 0, st, d.scenes, nscenes, d.start, d.PA, d.VA, d.rho, d.D2, \
-                       w.pack_a, w.pack_c, w.norm, d.n, w.pred, w.pairs, w.gelu_tab)
+                       w.pack_a, w.pack_c, w.norm, d.n, w.pred, w.pairs)
         if (sub == 1)
             NPRSPH_MFMA(1);
         else if (sub == 2)

@@ -346,12 +346,15 @@ int enqueue_build(sph_ctx* c)
     return SPH_OK;
 }
 
-int enqueue_step(sph_ctx* c)
+// One step on the sorted state.  write_aos = false leaves the AoS buffer to a
+// later step of the same call: no kernel of a step reads it.
+int enqueue_step(sph_ctx* c, bool write_aos)
 {
     int rc = enqueue_build(c);
     if (rc != SPH_OK) return rc;
     SPH_RUN(c, "density", launch_density(c->stream, c->d, false));
-    SPH_RUN(c, "force_integrate", launch_force(c->stream, c->d, true));
+    SPH_RUN(c, "force_integrate",
+            launch_force(c->stream, c->d, write_aos ? ForceMode::FusedAos : ForceMode::Fused));
     return SPH_OK;
 }
 
@@ -369,7 +372,7 @@ int capture_graph(sph_ctx* c, int which, int steps)
     hipGraph_t g = nullptr;
     SPH_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
     int rc = SPH_OK;
-    for (int k = 0; k < steps && rc == SPH_OK; ++k) rc = enqueue_step(c);
+    for (int k = 0; k < steps && rc == SPH_OK; ++k) rc = enqueue_step(c, k == steps - 1);
     hipError_t e = hipStreamEndCapture(c->stream, &g);
     if (rc != SPH_OK) {
         if (g) (void)hipGraphDestroy(g);
@@ -887,7 +890,7 @@ int sph_force(sph_ctx* c)
         SPH_RUN(c, "refresh_extras", launch_refresh(c->stream, c->d, false, true));
         c->rho_valid = true;
     }
-    SPH_RUN(c, "force", launch_force(c->stream, c->d, false));
+    SPH_RUN(c, "force", launch_force(c->stream, c->d, ForceMode::Pass));
     return SPH_OK;
 }
 
@@ -912,7 +915,7 @@ int sph_step(sph_ctx* c, int nsteps)
     if ((rc = ensure_b(c)) != SPH_OK) return rc;
     if (c->timing) {
         for (int k = 0; k < nsteps; ++k)
-            if ((rc = enqueue_step(c)) != SPH_OK) return rc;
+            if ((rc = enqueue_step(c, k == nsteps - 1)) != SPH_OK) return rc;
     } else {
         int big = nsteps / kUnroll, small = nsteps % kUnroll;
         if (big && !c->graph[1] && (rc = capture_graph(c, 1, kUnroll)) != SPH_OK) return rc;
@@ -1301,7 +1304,7 @@ int coarse_once(sph_ctx* c, bool use_network)
         }
         SPH_RUN(c, "mlp_messages", launch_coarse_mlp(c->stream, c->d, c->nscenes, c->mlp, 1));
     } else {
-        SPH_RUN(c, "force", launch_force(c->stream, c->d, false));
+        SPH_RUN(c, "force", launch_force(c->stream, c->d, ForceMode::Pass));
     }
     SPH_RUN(c, "coarse_integrate", launch_coarse_integrate(c->stream, c->d, c->mlp, !use_network));
     c->b_valid = true;

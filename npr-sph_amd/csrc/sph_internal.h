@@ -117,7 +117,10 @@ hipError_t launch_scan_down(hipStream_t s, const DeviceState& d);
 hipError_t launch_scatter(hipStream_t s, const DeviceState& d);
 hipError_t launch_rank_reorder(hipStream_t s, const DeviceState& d);
 hipError_t launch_density(hipStream_t s, const DeviceState& d, bool write_aos);
-hipError_t launch_force(hipStream_t s, const DeviceState& d, bool fuse_integrate);
+// Pass: force_comp.glsl alone (writes force into the AoS).  FusedAos / Fused:
+// force and integrate in one kernel, with or without the AoS record write.
+enum class ForceMode : int { Pass = 0, FusedAos = 1, Fused = 2 };
+hipError_t launch_force(hipStream_t s, const DeviceState& d, ForceMode mode);
 hipError_t launch_integrate_aos(hipStream_t s, const DeviceState& d);
 hipError_t launch_metrics(hipStream_t s, const DeviceState& d, int first_tile, int ntiles);
 

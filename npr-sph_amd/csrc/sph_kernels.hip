@@ -437,7 +437,9 @@ static_assert((kQueue & (kQueue - 1)) == 0 && kQueue >= 16, "queue ring size");
 // fills the second entry of a queue pair that a lane does not have.  P and V
 // hold sentinel + 1 elements.
 // FUSE additionally runs integrate_comp.glsl:35-81 on the new force, writes the
-// AoS record (pos, vel, force, extras.xy) and the next step's B state.
+// next step's B state and, unless aos is null, the AoS record (pos, vel,
+// force, extras.xy).  No kernel of a step reads the AoS, so only the last step
+// of a multi-step call needs to write it.
 // Occupancy: at 1 or 2 cells per h the kernel fits 96 VGPRs (5 waves per SIMD
 // instead of 4) with a few spills outside the neighbour loops, which is ~8%
 // faster; at 3 cells per h the larger stencil would spill inside them.
@@ -555,13 +557,14 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(SUB < 3 ?
         f[0] = fmaf(sc.pres_m, ax, fmaf(sc.visc_m, bx, rhoi * sc.grav[0]));
         f[1] = fmaf(sc.pres_m, ay, fmaf(sc.visc_m, by, rhoi * sc.grav[1]));
         f[2] = fmaf(sc.pres_m, az, fmaf(sc.visc_m, bz, rhoi * sc.grav[2]));
-        float* r = reinterpret_cast<float*>(aos + perm[s]);
         if constexpr (FUSE) {
             const float x[3] = {xi.x, xi.y, xi.z}, v[3] = {vi.x, vi.y, vi.z};
             float nx[3], nv[3];
             integrate_one(sc, x, v, f, rhoi, nx, nv);
             PN[s] = make_float4(nx[0], nx[1], nx[2], 0.f);
             VN[s] = make_float4(nv[0], nv[1], nv[2], 0.f);
+            if (!aos) return;  // a step whose record a later step of the same call overwrites
+            float* r = reinterpret_cast<float*>(aos + perm[s]);
             r[0] = nx[0];
             r[1] = nx[1];
             r[2] = nx[2];
@@ -574,6 +577,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(SUB < 3 ?
             r[12] = rhoi;
             r[13] = pi;
         } else {
+            float* r = reinterpret_cast<float*>(aos + perm[s]);
             r[8] = f[0];
             r[9] = f[1];
             r[10] = f[2];
@@ -772,7 +776,7 @@ inline dim3 gather_grid(const DeviceState& d)
 }
 
 template <int SUB, bool PRUNE, int BLOCK>
-hipError_t density_variant(hipStream_t s, const DeviceState& d, bool write_aos)
+hipError_t density_variant(hipStream_t s, const DeviceState& d, int write_aos)
 {
     hipLaunchKernelGGL((k_density<SUB, PRUNE, BLOCK>), gather_grid(d), dim3(BLOCK), 0, s, d.gtiles, d.scenes, d.start,
                        d.PA, d.rho, d.D2, reinterpret_cast<float*>(d.PA), reinterpret_cast<float*>(d.VA), d.perm,
@@ -780,19 +784,21 @@ hipError_t density_variant(hipStream_t s, const DeviceState& d, bool write_aos)
     return hipGetLastError();
 }
 
+// mode: ForcePass, or FusedStep with or without the AoS write.
 template <int SUB, bool PRUNE, int BLOCK>
-hipError_t force_variant(hipStream_t s, const DeviceState& d, bool fuse)
+hipError_t force_variant(hipStream_t s, const DeviceState& d, int mode)
 {
-    if (fuse)
+    if (mode != (int)ForceMode::Pass)
         hipLaunchKernelGGL((k_force<true, SUB, PRUNE, BLOCK>), gather_grid(d), dim3(BLOCK), 0, s, d.gtiles, d.scenes,
-                           d.start, d.PA, d.VA, d.rho, d.perm, d.aos, d.PB, d.VB, d.n, (uint32_t)d.ngtiles);
+                           d.start, d.PA, d.VA, d.rho, d.perm, mode == (int)ForceMode::FusedAos ? d.aos : nullptr,
+                           d.PB, d.VB, d.n, (uint32_t)d.ngtiles);
     else
         hipLaunchKernelGGL((k_force<false, SUB, PRUNE, BLOCK>), gather_grid(d), dim3(BLOCK), 0, s, d.gtiles, d.scenes,
                            d.start, d.PA, d.VA, d.rho, d.perm, d.aos, d.PB, d.VB, d.n, (uint32_t)d.ngtiles);
     return hipGetLastError();
 }
 
-using GatherFn = hipError_t (*)(hipStream_t, const DeviceState&, bool);
+using GatherFn = hipError_t (*)(hipStream_t, const DeviceState&, int);
 
 #define NPRSPH_VARIANTS(fn)                                                                    \
     {{{fn<1, false, 128>, fn<1, false, 256>}, {fn<1, true, 128>, fn<1, true, 256>}},           \
@@ -817,12 +823,12 @@ bool tuning_supported(const sph_tuning_t& t)
 
 hipError_t launch_density(hipStream_t s, const DeviceState& d, bool write_aos)
 {
-    return pick(kDensity, d.tuning)(s, d, write_aos);
+    return pick(kDensity, d.tuning)(s, d, write_aos ? 1 : 0);
 }
 
-hipError_t launch_force(hipStream_t s, const DeviceState& d, bool fuse_integrate)
+hipError_t launch_force(hipStream_t s, const DeviceState& d, ForceMode mode)
 {
-    return pick(kForce, d.tuning)(s, d, fuse_integrate);
+    return pick(kForce, d.tuning)(s, d, (int)mode);
 }
 
 hipError_t launch_integrate_aos(hipStream_t s, const DeviceState& d)

@@ -189,7 +189,10 @@ int sph_integrate(sph_ctx* ctx);
 
 /* nsteps x (rho_pres, force, integrate), the per-frame block of
  * Main.cpp:291-304, run as captured HIP graphs on cell-sorted state.  The AoS
- * buffer afterwards equals what the three passes would leave, bit for bit. */
+ * buffer afterwards equals what the three passes would leave, bit for bit.
+ * Steps that a later step of the same call follows may skip the AoS write
+ * and work on the sorted state alone; the buffer is complete once the call's
+ * work on the stream has finished. */
 int sph_step(sph_ctx* ctx, int nsteps);
 
 /* Device pointer of the AoS buffer (original index order, stride 64 bytes, pos

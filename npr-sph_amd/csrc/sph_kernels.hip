@@ -467,7 +467,14 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(SUB < 3 ?
         const float h = sc.h, h2 = sc.h2;
         float ax = 0.f, ay = 0.f, az = 0.f;  // pressure sum
         float bx = 0.f, by = 0.f, bz = 0.f;  // viscosity sum
-        int head = 0, tail = 0;  // this lane's queue entries are [head, tail) mod kQueue; head is even
+        // This lane's queue entries are [head, tail) mod kQueue, both counted
+        // in bytes of a queue row (kRow per entry), so an entry's LDS address
+        // is one and-or with the lane's column; head is an even entry.
+        static_assert((BLOCK & (BLOCK - 1)) == 0, "the ring wraps with the 32-bit byte counters");
+        constexpr uint32_t kRow = BLOCK * 4u, kWrap = (kQueue - 1) * kRow;
+        char* const qbase = reinterpret_cast<char*>(&queue[0][0]);
+        const uint32_t col = threadIdx.x * 4u;
+        uint32_t head = 0, tail = 0;
         // One neighbour's pressure and viscosity terms.  The hardware reciprocal
         // square root of r^2 = 0 (a coincident pair) is +inf, so r = 0*inf is
         // NaN, as normalize(0) in the reference.
@@ -493,30 +500,30 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(SUB < 3 ?
         // had one entry moves its tail past the unused slot.
         auto flush = [&](bool all) {
             for (;;) {
-                const int n = tail - head;
-                if (all ? !__any(n > 0)
-                        : (!__any(n > kQueue - 8) && __popcll(__ballot(n > 0)) < kDrainLanes))
+                const uint32_t n = tail - head;  // bytes
+                if (all ? !__any(n > 0u)
+                        : (!__any(n > (kQueue - 8) * kRow) && __popcll(__ballot(n > 0u)) < kDrainLanes))
                     break;
-                if (n > 0) {
-                    const uint32_t* e = &queue[head & (kQueue - 2)][threadIdx.x];
+                if (n > 0u) {
+                    const uint32_t* e = reinterpret_cast<const uint32_t*>(qbase + ((head & kWrap) | col));
                     const uint32_t j0 = e[0], e1 = e[BLOCK];
-                    const uint32_t j1 = n > 1 ? e1 : sentinel;
+                    const uint32_t j1 = n > kRow ? e1 : sentinel;
                     const float4 x0 = f4_load(rP, j0), v0 = f4_load(rV, j0);
                     const float4 x1 = f4_load(rP, j1), v1 = f4_load(rV, j1);
                     pair(x0, v0);
                     pair(x1, v1);
-                    head += 2;
-                    tail = max(tail, head);
+                    head += 2u * kRow;
+                    tail = n > kRow ? tail : head;
                 }
             }
         };
-        // Needs a free entry (tail - head < kQueue).  j is never this lane's
+        // Needs a free entry (fewer than kQueue queued).  j is never this lane's
         // own slot: the scan below leaves it out of the candidate ranges.
         auto consider = [&](const float4 xj, uint32_t j) {
             const float ddx = xi.x - xj.x, ddy = xi.y - xj.y, ddz = xi.z - xj.z;
             const float r2 = fmaf(ddx, ddx, fmaf(ddy, ddy, ddz * ddz));
-            queue[tail & (kQueue - 1)][threadIdx.x] = j;
-            tail += r2 < h2 ? 1 : 0;
+            *reinterpret_cast<uint32_t*>(qbase + ((tail & kWrap) | col)) = j;
+            tail += r2 < h2 ? kRow : 0u;
         };
         int cx = 0, cy = 0, cz = 0;
         float fx = 0.f, fy = 0.f, fz = 0.f;
@@ -538,7 +545,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(SUB < 3 ?
                                         uint32_t jc = j + first;
                                         for (; n >= 4u; n -= 4u, c += 4, jc += 4u) {
                                             const float4 x0 = c[0], x1 = c[1], x2 = c[2], x3 = c[3];
-                                            if (__any(tail - head > kQueue - 4)) flush(false);
+                                            if (__any(tail - head > (kQueue - 4) * kRow)) flush(false);
                                             consider(x0, jc);
                                             consider(x1, jc + 1u);
                                             consider(x2, jc + 2u);
@@ -546,7 +553,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(SUB < 3 ?
                                         }
                                         for (; n > 0u; --n, ++c, ++jc) {
                                             const float4 xj = *c;
-                                            if (__any(tail - head == kQueue)) flush(false);
+                                            if (__any(tail - head == kQueue * kRow)) flush(false);
                                             consider(xj, jc);
                                         }
                                     }

@@ -235,6 +235,34 @@ def test_regrid_right_after_enqueued_steps(pkg, orc, gpu_lib):
         assert eager.download().tobytes() == synced.download().tobytes()
 
 
+def test_event_timed_steps_equal_graph_steps(pkg, orc, gpu_lib):
+    """sph_set_timing runs the steps as separate launches bracketed by HIP events.
+    They give the same bits as the captured graphs, including the AoS record,
+    which only a call's last step writes: a three-pass frame issued between
+    two multi-step calls reads it."""
+    sc = pkg.scenes.dam10k()
+    p0 = _lattice(orc, sc)
+    b = pkg.sph.make_boundary(sc.lower, sc.upper)
+    with pkg.Simulation(sc.count, boundaries=b) as timed, pkg.Simulation(sc.count, boundaries=b) as graphed:
+        for sim in (timed, graphed):
+            sim.upload(p0)
+        timed.set_timing(True)
+        for sim in (timed, graphed):
+            sim.step(21)
+            _passes(sim)
+            sim.step(19)
+        assert timed.download().tobytes() == graphed.download().tobytes()
+        times = timed.kernel_times()
+        for k in ("hash_count", "scan_reduce", "scan_top", "scan_down", "scatter", "rank_reorder", "density"):
+            assert times[k][1] == 41, (k, times[k])  # 40 steps and the frame's rho_pres
+        assert times["force_integrate"][1] == 40
+        assert times["force"][1] == 1 and times["integrate"][1] == 1
+        assert all(ms > 0.0 for ms, _ in times.values())
+        timed.set_timing(False)
+        timed.step(5)
+        assert timed.kernel_times() == {}
+
+
 def test_set_constants_regrids(pkg, orc, gpu_lib):
     """The reference's smoothing slider (Main.cpp:244) changes h; the grid follows."""
     sc = pkg.scenes.dam10k()

@@ -417,7 +417,7 @@ def main(argv=None):
                     "profile": None if prof is None else prof.get("source"),
                     "profile_launch_ms": None if prof is None else prof.get("rocprof_avg_ms_timed_launches"),
                     "binding_roof": "VALU issue: the neighbour gathers are instruction-bound (about 10 VALU "
-                                    "instructions per candidate in the density scan, 13 in the force scan, "
+                                    "instructions per candidate in the density scan, 12 in the force scan, "
                                     "23 per neighbour in the force terms), with HBM mostly idle"}
     # Host readback of the whole AoS (sph_download into pageable numpy memory):
     # what a frame would cost a caller that takes every step's particles to the

@@ -339,6 +339,10 @@ def main(argv=None):
     ap.add_argument("--tune-steps", type=int, default=10)
     ap.add_argument("--no-learned", action="store_true", help="skip the learned coarse-step measurement")
     args = ap.parse_args(argv)
+    if args.steps < 1 or args.warmup < 0 or args.tune_steps < 1 or args.cpu_budget < 0:
+        ap.error("--steps and --tune-steps must be >= 1, --warmup and --cpu-budget >= 0")
+    if args.timing_steps is not None and args.timing_steps < 1:
+        ap.error("--timing-steps must be >= 1")
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (each GPU host here has one GPU, "
               "and the solver shards by spatial slab only in design)", file=sys.stderr)

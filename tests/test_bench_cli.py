@@ -23,6 +23,14 @@ def test_more_than_one_gpu_is_refused_with_a_message():
     assert "one GPU" in (r.stdout + r.stderr)
 
 
+def test_bad_step_counts_are_refused_before_any_gpu_work():
+    for args in (["--steps", "0"], ["--timing-steps", "0"], ["--warmup", "-1"]):
+        r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), *args], capture_output=True, text=True,
+                           timeout=120)
+        assert r.returncode == 2, (args, r.returncode, r.stderr)
+        assert ">=" in r.stderr, args
+
+
 def test_committed_profiles_match_the_default_tuning():
     """The bench line's roofline.traffic and VALU estimate come from these
     records; they apply only when the run's tuning equals the recorded one, and

@@ -339,7 +339,7 @@ int enqueue_build(sph_ctx* c)
     const DeviceState& d = c->d;
     SPH_RUN(c, "hash_count", launch_hash_count(c->stream, d));
     SPH_RUN(c, "scan_reduce", launch_scan_reduce(c->stream, d));
-    SPH_RUN(c, "scan_top", launch_scan_top(c->stream, d));
+    if (!scan_folds(d.total_cells)) SPH_RUN(c, "scan_top", launch_scan_top(c->stream, d));
     SPH_RUN(c, "scan_down", launch_scan_down(c->stream, d));
     SPH_RUN(c, "scatter", launch_scatter(c->stream, d));
     SPH_RUN(c, "rank_reorder", launch_rank_reorder(c->stream, d));

@@ -237,8 +237,12 @@ __global__ __launch_bounds__(1024) void k_scan_top(uint32_t* __restrict__ bsum, 
 // and start[n] receives the total.  Each thread owns kScanItems consecutive
 // cells and moves them as 16-byte vectors where the whole vector is in range
 // (count and start are 256-byte aligned, kScanItems is a multiple of 4).
+// With fold, bsum holds the raw block sums and each block adds up those
+// before it (grids of at most kScanFoldBlocks blocks, so at most that many
+// reads per block); otherwise k_scan_top has already scanned them.
 __global__ __launch_bounds__(kScanThreads) void k_scan_down(uint32_t* __restrict__ count, uint32_t n,
-                                                            const uint32_t* __restrict__ bsum, uint32_t* __restrict__ start)
+                                                            const uint32_t* __restrict__ bsum, uint32_t* __restrict__ start,
+                                                            uint32_t fold)
 {
     static_assert(kScanItems % 4 == 0, "vector width");
     const uint32_t base = blockIdx.x * (uint32_t)kScanBlock + threadIdx.x * (uint32_t)kScanItems;
@@ -261,8 +265,16 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_down(uint32_t* __restrict
     }
 #pragma unroll
     for (int k = 0; k < kScanItems; ++k) sum += v[k];
+    uint32_t prefix;
+    if (fold) {
+        uint32_t part = 0;
+        for (uint32_t b = threadIdx.x; b < blockIdx.x; b += kScanThreads) part += bsum[b];
+        prefix = block_sum<kScanThreads>(part);
+    } else {
+        prefix = bsum[blockIdx.x];
+    }
     uint32_t total;
-    uint32_t run = bsum[blockIdx.x] + block_excl_scan<kScanThreads>(sum, total);
+    uint32_t run = prefix + block_excl_scan<kScanThreads>(sum, total);
     uint32_t out[kScanItems];
 #pragma unroll
     for (int k = 0; k < kScanItems; ++k) {
@@ -755,7 +767,7 @@ hipError_t launch_scan_top(hipStream_t s, const DeviceState& d)
 hipError_t launch_scan_down(hipStream_t s, const DeviceState& d)
 {
     hipLaunchKernelGGL(k_scan_down, dim3(scan_blocks(d.total_cells)), dim3(kScanThreads), 0, s, d.count, d.total_cells,
-                       d.bsum, d.start);
+                       d.bsum, d.start, scan_folds(d.total_cells) ? 1u : 0u);
     return hipGetLastError();
 }
 

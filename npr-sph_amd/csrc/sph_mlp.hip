@@ -564,7 +564,7 @@ __global__ __launch_bounds__(256) void k_mlp_pack(const float* __restrict__ para
     }
 }
 
-constexpr int kHitCap = 512;  // hit list entries per wave
+constexpr int kHitCap = 192;  // hit list entries per wave
 
 // Summed messages of every slot on MFMA: one wave per target particle.  The
 // wave scans the target's stencil rows 64 candidates at a time, compacts the
@@ -579,32 +579,51 @@ __global__ __launch_bounds__(256) void k_coarse_mlp_mfma(const SceneDev* __restr
                                                          unsigned long long* __restrict__ pair_count)
 {
     __shared__ uint32_t hits_all[4][kHitCap];
+    // The network's MFMA operands (weights and biases), staged once per
+    // workgroup as per-lane images and read back at each MFMA, so that a
+    // wave's registers hold only its working set (more waves per SIMD).
+    __shared__ bf16x8 s_a1[4][64], s_a2[8][64], s_a3[2][64];
+    // Biases once each: b1[64], b2[64], b3 padded with zeros to 16.  A lane's
+    // C operand for MFMA m is the float4 at 16m + 4(lane/16) (a 16-lane
+    // broadcast), the same values as its packed image.
+    __shared__ f32x4 s_b[16 + 16 + 4];
     uint32_t wave_pairs = 0;  // pairs this wave evaluated, added to *pair_count at the end
     const int lane = (int)(threadIdx.x & 63);
     const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     uint32_t* hits = hits_all[wv];
-    bf16x8 a1[4], a2[4][2], a3[2];
-    f32x4 c1[4], c2[4], c3;
+    for (int i = (int)threadIdx.x; i < 4 * 64; i += 256) {
+        const int m = i / 64, l = i % 64;
+        bf16x8 a;
 #pragma unroll
-    for (int m = 0; m < 4; ++m) {
-#pragma unroll
-        for (int e = 0; e < 8; ++e) a1[m][e] = pa[kPackA1 + (m * 64 + lane) * 8 + e];
-#pragma unroll
-        for (int s = 0; s < 2; ++s)
-#pragma unroll
-            for (int e = 0; e < 8; ++e) a2[m][s][e] = pa[kPackA2 + ((m * 2 + s) * 64 + lane) * 8 + e];
-#pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            c1[m][e] = pc[kPackC1 + (m * 64 + lane) * 4 + e];
-            c2[m][e] = pc[kPackC2 + (m * 64 + lane) * 4 + e];
-        }
+        for (int e = 0; e < 8; ++e) a[e] = pa[kPackA1 + (m * 64 + l) * 8 + e];
+        s_a1[m][l] = a;
     }
+    if (threadIdx.x < 36) {
+        // float4 q of s_b: bias rows 4q..4q+3 of b1 (q < 16), b2 (q < 32) or b3,
+        // read from the packed image of lane 16*(q%4) of MFMA q/4.
+        const int q = (int)threadIdx.x, base = q < 16 ? kPackC1 : (q < 32 ? kPackC2 : kPackC3);
+        const int r = q < 32 ? q % 16 : q - 32;
+        const int img = q < 32 ? ((r / 4) * 64 + 16 * (r % 4)) * 4 : 16 * r * 4;
+        f32x4 c;
 #pragma unroll
-    for (int s = 0; s < 2; ++s)
+        for (int e = 0; e < 4; ++e) c[e] = pc[base + img + e];
+        s_b[q] = c;
+    }
+    for (int i = (int)threadIdx.x; i < 2 * 64; i += 256) {
+        const int m = i / 64, l = i % 64;
+        bf16x8 a;
 #pragma unroll
-        for (int e = 0; e < 8; ++e) a3[s][e] = pa[kPackA3 + (s * 64 + lane) * 8 + e];
+        for (int e = 0; e < 8; ++e) a[e] = pa[kPackA3 + (m * 64 + l) * 8 + e];
+        s_a3[m][l] = a;
+    }
+    for (int i = (int)threadIdx.x; i < 8 * 64; i += 256) {
+        const int ms = i / 64, l = i % 64;
+        bf16x8 a;
 #pragma unroll
-    for (int e = 0; e < 4; ++e) c3[e] = pc[kPackC3 + lane * 4 + e];
+        for (int e = 0; e < 8; ++e) a[e] = pa[kPackA2 + (ms * 64 + l) * 8 + e];
+        s_a2[ms][l] = a;
+    }
+    __syncthreads();
 
     const uint32_t nwaves = gridDim.x * 4u;
     for (uint32_t si = blockIdx.x * 4u + (uint32_t)wv; si < n; si += nwaves) {
@@ -621,6 +640,9 @@ __global__ __launch_bounds__(256) void k_coarse_mlp_mfma(const SceneDev* __restr
         auto run_tiles = [&](int count, bool partial) {
             const int ntiles = partial ? (count + 15) / 16 : count / 16;
             for (int tt = 0; tt < ntiles; ++tt) {
+                // Keeps the staged operands' LDS reads inside the loop (no
+                // instruction is emitted).
+                asm volatile("" ::: "memory");
                 const int pidx = tt * 16 + (lane & 15);
                 const bool valid = pidx < count;
                 const uint32_t j = valid ? hits[pidx] : s;
@@ -644,7 +666,8 @@ __global__ __launch_bounds__(256) void k_coarse_mlp_mfma(const SceneDev* __restr
                 }
                 f32x4 h1[4];
 #pragma unroll
-                for (int m = 0; m < 4; ++m) h1[m] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1[m], bx, c1[m], 0, 0, 0);
+                for (int m = 0; m < 4; ++m)
+                    h1[m] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(s_a1[m][lane], bx, s_b[4 * m + lane / 16], 0, 0, 0);
                 bf16x8 b2[2];
 #pragma unroll
                 for (int ks = 0; ks < 2; ++ks)
@@ -659,8 +682,8 @@ __global__ __launch_bounds__(256) void k_coarse_mlp_mfma(const SceneDev* __restr
                 f32x4 h2[4];
 #pragma unroll
                 for (int m = 0; m < 4; ++m) {
-                    h2[m] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2[m][0], b2[0], c2[m], 0, 0, 0);
-                    h2[m] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2[m][1], b2[1], h2[m], 0, 0, 0);
+                    h2[m] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(s_a2[2 * m][lane], b2[0], s_b[16 + 4 * m + lane / 16], 0, 0, 0);
+                    h2[m] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(s_a2[2 * m + 1][lane], b2[1], h2[m], 0, 0, 0);
                 }
                 bf16x8 b3[2];
 #pragma unroll
@@ -673,8 +696,8 @@ __global__ __launch_bounds__(256) void k_coarse_mlp_mfma(const SceneDev* __restr
                             b3[ks][4 * mm + e] = g.x;
                             b3[ks][4 * mm + e + 1] = g.y;
                         }
-                f32x4 o = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a3[0], b3[0], c3, 0, 0, 0);
-                o = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a3[1], b3[1], o, 0, 0, 0);
+                f32x4 o = __builtin_amdgcn_mfma_f32_16x16x32_bf16(s_a3[0][lane], b3[0], s_b[32 + lane / 16], 0, 0, 0);
+                o = __builtin_amdgcn_mfma_f32_16x16x32_bf16(s_a3[1][lane], b3[1], o, 0, 0, 0);
                 if (lane < 16 && valid) {
                     acc0 += o[0];
                     acc1 += o[1];

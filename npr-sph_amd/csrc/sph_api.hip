@@ -913,15 +913,26 @@ int sph_step(sph_ctx* c, int nsteps)
     int rc = set_device(c);
     if (rc != SPH_OK) return rc;
     if ((rc = ensure_b(c)) != SPH_OK) return rc;
-    if (c->timing) {
-        for (int k = 0; k < nsteps; ++k)
-            if ((rc = enqueue_step(c, k == nsteps - 1)) != SPH_OK) return rc;
-    } else {
-        int big = nsteps / kUnroll, small = nsteps % kUnroll;
-        if (big && !c->graph[1] && (rc = capture_graph(c, 1, kUnroll)) != SPH_OK) return rc;
-        if (small && !c->graph[0] && (rc = capture_graph(c, 0, 1)) != SPH_OK) return rc;
+    auto run_steps = [&]() -> int {
+        int r;
+        if (c->timing) {
+            for (int k = 0; k < nsteps; ++k)
+                if ((r = enqueue_step(c, k == nsteps - 1)) != SPH_OK) return r;
+            return SPH_OK;
+        }
+        const int big = nsteps / kUnroll, small = nsteps % kUnroll;
+        if (big && !c->graph[1] && (r = capture_graph(c, 1, kUnroll)) != SPH_OK) return r;
+        if (small && !c->graph[0] && (r = capture_graph(c, 0, 1)) != SPH_OK) return r;
         for (int k = 0; k < big; ++k) SPH_HIP(c, hipGraphLaunch(c->graph[1], c->stream));
         for (int k = 0; k < small; ++k) SPH_HIP(c, hipGraphLaunch(c->graph[0], c->stream));
+        return SPH_OK;
+    };
+    if ((rc = run_steps()) != SPH_OK) {
+        // Steps already enqueued may have advanced the sorted state past the
+        // AoS buffer, which holds the last step that wrote it: the next call
+        // rebuilds from the buffer.
+        invalidate_state(c);
+        return rc;
     }
     c->b_valid = true;
     c->a_valid = false;

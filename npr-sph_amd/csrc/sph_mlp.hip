@@ -568,9 +568,12 @@ constexpr int kHitCap = 192;  // hit list entries per wave
 
 // Summed messages of every slot on MFMA: one wave per target particle.  The
 // wave scans the target's stencil rows 64 candidates at a time, compacts the
-// hits into an LDS list, and runs the network on 16-pair tiles.
+// hits into an LDS list, and runs the network on 16-pair tiles.  With the
+// network's operands in LDS a wave fits 64 registers, so 8 waves share each
+// SIMD (18 KB of LDS per 4-wave workgroup): the GELU arithmetic of one wave
+// issues while others wait on their MFMA results and loads.
 template <int SUB>
-__global__ __launch_bounds__(256) void k_coarse_mlp_mfma(const SceneDev* __restrict__ scenes, int nscenes,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k_coarse_mlp_mfma(const SceneDev* __restrict__ scenes, int nscenes,
                                                          const uint32_t* __restrict__ start,
                                                          const float4* __restrict__ P, const float4* __restrict__ V,
                                                          const float* __restrict__ rho, const float2* __restrict__ D2,

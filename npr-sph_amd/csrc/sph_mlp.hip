@@ -725,34 +725,60 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k
             const int z0 = cz > SUB ? cz - SUB : 0, z1 = cz < dz - SUB ? cz + SUB : dz - 1;
             const int y0 = cy > SUB ? cy - SUB : 0, y1 = cy < dy - SUB ? cy + SUB : dy - 1;
             constexpr float kReach2 = (float)(SUB * SUB);
-            for (int z = z0; z <= z1; ++z) {
+            // Lane r < nrows finds stencil row r (z-major, y ascending, as the
+            // plain walk orders them): its slot range [rb, rb + rl), empty when
+            // the row lies beyond h.  At most (2 SUB + 1)^2 <= 49 rows.
+            static_assert((2 * SUB + 1) * (2 * SUB + 1) <= 64, "one lane per stencil row");
+            const int ny = y1 - y0 + 1, nrows = (z1 - z0 + 1) * ny;
+            uint32_t rb = 0, rl = 0;
+            if (lane < nrows) {
+                const int z = z0 + lane / ny, y = y0 + lane % ny;
                 const float gz = z > cz ? (float)z - fz : (z < cz ? fz - (float)(z + 1) : 0.f);
-                for (int y = y0; y <= y1; ++y) {
-                    const float gy = y > cy ? (float)y - fy : (y < cy ? fy - (float)(y + 1) : 0.f);
-                    const float rem = kReach2 - fmaf(gy, gy, gz * gz);
-                    if (!(rem > 0.f)) continue;
+                const float gy = y > cy ? (float)y - fy : (y < cy ? fy - (float)(y + 1) : 0.f);
+                const float rem = kReach2 - fmaf(gy, gy, gz * gz);
+                if (rem > 0.f) {
                     const float w = __builtin_amdgcn_sqrtf(rem);
                     const int x0 = (int)fmaxf(fminf(floorf(fx - w), (float)(dx - 1)), (float)xa);
                     const int x1 = (int)fminf(fmaxf(floorf(fx + w), 0.f), (float)xb);
                     const uint32_t row = sc.cell_base + ((uint32_t)z * (uint32_t)dy + (uint32_t)y) * (uint32_t)dx;
-                    const uint32_t b = start[row + (uint32_t)x0], e = start[row + (uint32_t)(x1 + 1)];
-                    for (uint32_t base = b; base < e; base += 64u) {
-                        const uint32_t j = base + (uint32_t)lane;
-                        bool hit = false;
-                        if (j < e && j != s) {
-                            const float4 xj = P[j];
-                            const float ddx = xi.x - xj.x, ddy = xi.y - xj.y, ddz = xi.z - xj.z;
-                            hit = fmaf(ddx, ddx, fmaf(ddy, ddy, ddz * ddz)) < h2;
-                        }
-                        const uint64_t mask = __ballot(hit);
-                        const int posn = nh + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
-                                                                            __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
-                        if (hit) hits[posn] = j;
-                        nh += __popcll(mask);
-                        wave_pairs += (uint32_t)__popcll(mask);
-                        if (nh > kHitCap - 64) nh = run_tiles(nh, false);
-                    }
+                    rb = start[row + (uint32_t)x0];
+                    rl = start[row + (uint32_t)(x1 + 1)] - rb;
                 }
+            }
+            // The rows laid end to end: row r covers positions [incl - rl, incl).
+            uint32_t incl = rl;
+#pragma unroll
+            for (int d = 1; d < 64; d <<= 1) {
+                const uint32_t t = __shfl_up(incl, d);
+                if (lane >= d) incl += t;
+            }
+            const uint32_t total = (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
+            // 64 positions per round, so rounds are full across row ends; the
+            // hits keep the row-by-row, slot-ascending order.
+            int r0 = 0;  // first row that ends after the round's first position
+            for (uint32_t base = 0; base < total; base += 64u) {
+                while ((uint32_t)__builtin_amdgcn_readlane((int)incl, r0) <= base) ++r0;
+                const uint32_t g = base + (uint32_t)lane;
+                uint32_t j = 0;
+                for (int q = r0; q < nrows; ++q) {
+                    const uint32_t qe = (uint32_t)__builtin_amdgcn_readlane((int)incl, q);
+                    const uint32_t qs = qe - (uint32_t)__builtin_amdgcn_readlane((int)rl, q);
+                    if (qs >= base + 64u) break;
+                    if (g >= qs && g < qe) j = (uint32_t)__builtin_amdgcn_readlane((int)rb, q) + (g - qs);
+                }
+                bool hit = false;
+                if (g < total && j != s) {
+                    const float4 xj = P[j];
+                    const float ddx = xi.x - xj.x, ddy = xi.y - xj.y, ddz = xi.z - xj.z;
+                    hit = fmaf(ddx, ddx, fmaf(ddy, ddy, ddz * ddz)) < h2;
+                }
+                const uint64_t mask = __ballot(hit);
+                const int posn = nh + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
+                                                                    __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+                if (hit) hits[posn] = j;
+                nh += __popcll(mask);
+                wave_pairs += (uint32_t)__popcll(mask);
+                if (nh > kHitCap - 64) nh = run_tiles(nh, false);
             }
         }
         run_tiles(nh, true);

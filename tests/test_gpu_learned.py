@@ -229,6 +229,35 @@ def test_predict_mfma_matches_fp32(pkg, orc, ref, gpu_lib):
         assert np.linalg.norm(b - a) / np.linalg.norm(a) < 2e-2
 
 
+@pytest.mark.parametrize("sub", [1, 3])
+def test_predict_mfma_every_stencil_size(pkg, orc, ref, gpu_lib, sub):
+    """The message kernel at 1 and 3 cells per h (9 and 49 stencil rows, one
+    lane each) matches the FP32 path to BF16 accuracy and the 2-cells-per-h
+    messages up to the summation order, also where hit lists overflow."""
+    sim, sc = _scene(pkg, orc, n=4000, steps=5)
+    params = ref.init_params(19)
+    params[ref.NPARAMS - 195:] += np.random.default_rng(20).normal(0, 0.05, 195).astype(np.float32)
+    sim.mlp_set(params, _norm(pkg))
+    p2 = sim.mlp_predict(path=1)
+    sim.set_tuning(cells_per_h=sub)
+    p32 = sim.mlp_predict(path=0)
+    pbf = sim.mlp_predict(path=1)
+    assert np.all(np.isfinite(pbf))
+    assert np.linalg.norm(pbf - p32) / np.linalg.norm(p32) < 2e-2
+    assert np.linalg.norm(pbf - p2) / np.linalg.norm(p2) < 1e-4
+    sim.close()
+    n = 700
+    with pkg.Simulation(n) as dense:
+        p = orc.particles(n)
+        p["pos"][:, :3] = np.random.default_rng(1).uniform(0.0, 0.008, (n, 3)).astype(np.float32)
+        dense.upload(p)
+        dense.set_tuning(cells_per_h=sub)
+        dense.mlp_set(params, _norm(pkg))
+        a = dense.mlp_predict(path=0)
+        b = dense.mlp_predict(path=1)
+        assert np.linalg.norm(b - a) / np.linalg.norm(a) < 2e-2
+
+
 def test_trained_network_beats_no_network(pkg, orc, gpu_lib):
     """The committed weights (trained on the GPU on 200 dam1m rollouts) predict
     the 4-step mean acceleration of held-out states better than the method

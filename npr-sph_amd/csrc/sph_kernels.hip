@@ -26,19 +26,13 @@ constexpr int kWave = 64;
 __device__ __forceinline__ int lane_id() { return (int)__lane_id(); }
 
 // Workgroups are dealt to the 8 XCDs round-robin (workgroup b runs on XCD
-// b % 8), and each XCD has its own L2.  The gathers map workgroup b to tile
-// (b % 8) * per + b / 8, so every XCD walks one contiguous eighth of the
-// cell-sorted slots and its L2 holds that region's neighbours.  The launch
-// has 8*per workgroups; the ones past ntiles exit.
+// b % 8), and each XCD has its own L2.  The gathers give each XCD one
+// contiguous eighth of the cell-sorted slots (for_each_gather_tile), so its
+// L2 holds that region's neighbours.
 constexpr int kXcds = 8;
 
+// Tiles per XCD.
 __host__ __device__ __forceinline__ uint32_t xcd_per(uint32_t ntiles) { return (ntiles + kXcds - 1) / kXcds; }
-
-__device__ __forceinline__ uint32_t xcd_tile(uint32_t ntiles)
-{
-    const uint32_t per = xcd_per(ntiles);
-    return (blockIdx.x % kXcds) * per + blockIdx.x / kXcds;
-}
 
 __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v)
 {

@@ -5,13 +5,24 @@ grid build + rho_pres + force + integrate over every particle.  The default
 workload is dam1m: 1,000,000 particles, a jittered 100^3 dam-break lattice at
 spacing 0.5h with the reference's constants, initialised on the device.
 
+The timed state is the collapsed dam: --settle steps (default 400) from the
+seeded lattice, then --warmup steps, all untimed.  Before the clock starts the
+tuning is chosen (autotuner) and both step graphs are captured (sph_step with
+0 steps).  The timed region is --reps repetitions (default 5) of --steps
+steps, each restarted from the same state; `value` uses their median.
+
 Printed JSON line (one):
-  value        particle-steps/s over the timed steps (inputs resident in HBM)
-  repeats      median, min and max ms/step of 5 more blocks of steps
+  value        particle-steps/s, median of the timed repetitions (inputs
+               resident in HBM)
+  timed        every repetition's ms/step, with median, min and max
+  early_phase  the same step count from the lattice before the collapse
+               (warmup+1 onwards), for comparison; not `value`
   roofline     the dominant kernel: algorithmic bytes per launch / its mean
                duration over the timed region's steps (replayed from the
                region's starting state with HIP events on the solver's
-               stream) vs 8 TB/s
+               stream) vs 8 TB/s; HBM traffic and counter-derived VALU
+               utilisation from the committed rocprofv3 record of the same
+               workload, state and tuning
   three_passes the reference's ungraphed three-call surface per frame and the
                streaming integrate kernel's HBM rate
   cpu_baseline the FP64 oracle (grid form, OpenMP) on this host, on a bounded
@@ -48,9 +59,11 @@ STEP_BYTES = 20 + 44 + 64  # the three reference passes, 128 B per particle-step
 # The streaming integrate kernel over the AoS buffer moves whole float4s: the
 # 64-byte record read, pos and vel (32 bytes) written.
 INTEGRATE_MOVED_BYTES = 64 + 32
-# Repeated blocks of steps after the timed region (spread of the step time).
-REPEAT_BLOCKS = 5
-REPEAT_BLOCK_STEPS = 200
+# Untimed steps from the lattice to the collapsed state the timed region runs on
+# (the dam-break's collapse settles within about 300 steps).
+SETTLE_STEPS = 400
+# Timed repetitions of --steps steps from the same state (median reported).
+TIMED_REPS = 5
 # Coarse steps of the learned form timed from the plain run's warmed state,
 # beside the plain steps they stand in for.
 LEARNED_WINDOW = 10
@@ -147,24 +160,33 @@ def density_timeline(sim, total_fine, every_fine, coarse=None):
     return np.array(out), m.kinetic, np.array(m.com)
 
 
-def learned_form(pkg, scenes, warmup, fine_steps, plain_ms_per_step, weights_path=None):
-    """Learned coarse step on the same workload: timed run of fine_steps/4
-    coarse steps (judge every 50, drift limit 0.01*rho0), then untimed
-    mean-density timelines of plain, learned and no-network coarse runs from
-    the same warmed state."""
+# The learned form's gate: every JUDGE_EVERY-th coarse step is also run as 4
+# plain steps from the same state; a relative mean-density drift above
+# DRIFT_LIMIT keeps the plain result, and the next JUDGE_EVERY - 1 coarse steps
+# then run as plain steps before the network is judged again.
+JUDGE_EVERY = 50
+DRIFT_LIMIT = 0.01
+
+
+def learned_form(pkg, scenes, state, first_step, fine_steps, plain_ms_per_step, weights_path=None):
+    """Learned coarse step on the same workload from the timed region's
+    starting state: timed run of fine_steps/4 coarse steps under the gate
+    (JUDGE_EVERY, DRIFT_LIMIT, cooldown JUDGE_EVERY - 1), the network path
+    alone over a short window, then untimed mean-density timelines of plain,
+    learned and no-network coarse runs from the same state."""
     sys.path.insert(0, os.path.join(ROOT, "npr-sph_amd"))
     import train_surrogate
 
     params, norm, meta = train_surrogate.load_weights(weights_path)
-    ncoarse = fine_steps // 4
-    rho0 = 1000.0
-    judge = dict(use_network=True, judge_every=50, drift_limit=0.01 * rho0, cooldown=0)
+    ncoarse = max(1, fine_steps // 4)
+    judge = dict(use_network=True, judge_every=JUDGE_EVERY, drift_limit=DRIFT_LIMIT, cooldown=JUDGE_EVERY - 1)
     sim = make_sim(pkg, scenes)
-    sim.step(warmup)
     sim.mlp_set(params, norm)
-    s0 = sim.download()
+    s0 = state
+    sim.upload(s0)
     sim.coarse_step(1, **judge)  # kernel warm-up
     sim.upload(s0)
+    sim.prepare()
     sim.synchronize()
     t0 = time.perf_counter()
     st = sim.coarse_step(ncoarse, **judge)
@@ -189,24 +211,23 @@ def learned_form(pkg, scenes, warmup, fine_steps, plain_ms_per_step, weights_pat
                 "binding_roof": "VALU issue: 128 GELU evaluations per pair run on the vector ALUs between the "
                                 "MFMA layers"}
     # The same window from the same state for both forms: LEARNED_WINDOW coarse
-    # steps and the 4x as many plain steps they stand in for.  The whole-run
-    # figure above also carries the learned rollout's drift (its pair count
-    # grows as it compresses the fluid further than plain SPH does).
+    # steps of the network alone (no judge) and the 4x as many plain steps
+    # they stand in for.
     sim.upload(s0)
     sim.synchronize()
     t1 = time.perf_counter()
-    sim.coarse_step(LEARNED_WINDOW, **judge)
+    sim.coarse_step(LEARNED_WINDOW, use_network=True, judge_every=0, drift_limit=0.0, cooldown=0)
     sim.synchronize()
     win_coarse_ms = 1e3 * (time.perf_counter() - t1) / LEARNED_WINDOW
     sim.upload(s0)
-    sim.step(1)
+    sim.prepare()
     sim.synchronize()
     t1 = time.perf_counter()
     sim.step(4 * LEARNED_WINDOW)
     sim.synchronize()
     win_plain_ms = 1e3 * (time.perf_counter() - t1) / (4 * LEARNED_WINDOW)
-    window = {"coarse_steps": LEARNED_WINDOW, "ms_per_coarse_step": win_coarse_ms, "plain_ms_per_step": win_plain_ms,
-              "speedup_vs_plain": 4.0 * win_plain_ms / win_coarse_ms}
+    window = {"coarse_steps": LEARNED_WINDOW, "ms_per_coarse_step_network_only": win_coarse_ms,
+              "plain_ms_per_step": win_plain_ms, "speedup_vs_plain_network_only": 4.0 * win_plain_ms / win_coarse_ms}
     progress(f"learned: {ncoarse} coarse steps in {el:.2f} s; accuracy timelines")
     # accuracy: mean density every `every` fine steps of simulated time
     every = 20 if n <= 2_000_000 else 100
@@ -243,8 +264,11 @@ def learned_form(pkg, scenes, warmup, fine_steps, plain_ms_per_step, weights_pat
         "same_state_window": window,
         "mfma_roofline": mfma,
         "judge": st.as_dict(),
-        "drift_definition": "judge every 50 coarse steps: |mean rho(1 coarse step) - mean rho(4 plain steps)| from the "
-                            "same state; above 0.01*rho0 the plain result is kept",
+        "first_step": first_step,
+        "drift_definition": f"judge every {JUDGE_EVERY} coarse steps: |mean rho(1 coarse step) - mean rho(4 plain "
+                            f"steps)| / mean rho(4 plain steps) from the same state; above {DRIFT_LIMIT} the plain "
+                            f"result is kept and the next {JUDGE_EVERY - 1} coarse steps run plain before the next "
+                            "judge",
         "mean_density_rel_error_vs_plain": err(learned),
         "no_network_mean_density_rel_error_vs_plain": err(ablation),
         "final_kinetic_rel_error_vs_plain": fin(abs(ke_l - ke_p) / ke_p),
@@ -256,17 +280,20 @@ def learned_form(pkg, scenes, warmup, fine_steps, plain_ms_per_step, weights_pat
     }
 
 
-def profile_record(workload, kernel, tuning):
-    """The committed rocprofv3 record of `kernel` for this workload and tuning
-    (profiles/traffic_<workload>.json), or None: HBM bytes per launch from
-    FETCH_SIZE / WRITE_SIZE calibrated by a plain copy, and the VALU issue
-    estimate (4 SIMD cycles per wave64 SQ_INSTS_VALU instruction over the
-    dispatch duration x 1024 SIMDs x 2.4 GHz)."""
+def profile_record(workload, kernel, tuning, settle):
+    """The committed rocprofv3 record of `kernel` for this workload, settled
+    state and tuning (profiles/traffic_<workload>.json), or None.  It holds the
+    kernel's HBM bytes per launch (FETCH_SIZE and WRITE_SIZE, each calibrated
+    by the same counter over a plain copy of known size) and counter-derived
+    utilisations: VALU busy (SQ_ACTIVE_INST_VALU over CU count x
+    GRBM_GUI_ACTIVE, rocprofv3's VALUBusy), the active-lane fraction of VALU
+    instructions (SQ_THREAD_CYCLES_VALU over 64 x SQ_ACTIVE_INST_VALU,
+    rocprofv3's VALUUtilization) and the useful FP32 rate of the flop model."""
     path = os.path.join(ROOT, "profiles", f"traffic_{workload}.json")
     if not os.path.exists(path):
         return None
     rec = json.load(open(path))
-    if rec.get("kernel") != kernel or rec.get("tuning") != tuning:
+    if rec.get("kernel") != kernel or rec.get("tuning") != tuning or rec.get("settle") != settle:
         return None
     return rec
 
@@ -324,23 +351,48 @@ def copy_probe(mbytes: int, reps: int = 20):
     print(json.dumps({"copy_probe_MiB": mbytes, "GBps_rd_wr": 2 * n * 4 * reps / el / 1e9}))
 
 
+def timed_repetitions(sim, region_start, steps, reps):
+    """`reps` repetitions of `steps` graph-launched steps, each restarted from
+    `region_start`: upload, sph_step(ctx, 0) (sorted state and both step graphs
+    ready), synchronise, then the timed steps bracketed by synchronisation.
+    Every repetition runs the same trajectory bit for bit.  Returns the
+    elapsed seconds of each."""
+    out = []
+    for _ in range(reps):
+        sim.upload(region_start)
+        sim.prepare()
+        sim.synchronize()
+        t0 = time.perf_counter()
+        sim.step(steps)
+        sim.synchronize()
+        out.append(time.perf_counter() - t0)
+    return out
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--workload", default="dam1m")
+    ap.add_argument("--settle", type=int, default=SETTLE_STEPS,
+                    help="untimed steps from the initial lattice that bring the dam-break to the collapsed state "
+                         "the timed region runs on (part of the workload's definition; --warmup steps follow)")
+    ap.add_argument("--reps", type=int, default=TIMED_REPS,
+                    help="timed repetitions of --steps steps, each restarted from the same state (median reported)")
     ap.add_argument("--timing-steps", type=int, default=None,
                     help="steps of the timed region replayed with per-kernel HIP events (default: all of them)")
     ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU oracle work (0 = skip)")
     ap.add_argument("--copy-probe", type=int, default=0, help="MiB: run only a plain copy kernel")
     ap.add_argument("--tune", default="auto",
-                    help="'auto' (autotune on the warmed state, untimed), 'default', or cells_per_h,prune,block,bpx")
+                    help="'auto' (autotune on the settled state, untimed), 'default', or cells_per_h,prune,block,bpx")
     ap.add_argument("--tune-steps", type=int, default=10)
     ap.add_argument("--no-learned", action="store_true", help="skip the learned coarse-step measurement")
+    ap.add_argument("--no-early", action="store_true", help="skip the pre-collapse (early phase) figure")
     args = ap.parse_args(argv)
-    if args.steps < 1 or args.warmup < 0 or args.tune_steps < 1 or args.cpu_budget < 0:
-        ap.error("--steps and --tune-steps must be >= 1, --warmup and --cpu-budget >= 0")
+    if args.steps < 1 or args.warmup < 0 or args.tune_steps < 1 or args.cpu_budget < 0 or args.settle < 0 \
+            or args.reps < 1:
+        ap.error("--steps, --reps and --tune-steps must be >= 1, --warmup, --settle and --cpu-budget >= 0")
     if args.timing_steps is not None and args.timing_steps < 1:
         ap.error("--timing-steps must be >= 1")
     if args.gpus != 1:
@@ -358,46 +410,52 @@ def main(argv=None):
     sim = make_sim(pkg, scenes)
     if args.tune not in ("auto", "default"):
         sim.set_tuning(*[int(v) for v in args.tune.split(",")])
-    sim.step(args.warmup)
-    sim.synchronize()
-    warm0 = sim.download()[:scenes[0].count] if args.cpu_budget > 0 else None
-    if args.tune == "auto":
-        sim.autotune(trial_steps=args.tune_steps)
-    tuning = sim.tuning().as_dict()
-    sim.step(1)  # capture the step graphs of the chosen tuning outside the timed region
-    sim.synchronize()
-    region_start = sim.download()  # the timed region's starting state, replayed below
-    progress(f"tuning {tuning}; timing {args.steps} steps")
-    t0 = time.perf_counter()
-    sim.step(args.steps)
-    sim.synchronize()
-    elapsed = time.perf_counter() - t0
-    ms_per_step = 1e3 * elapsed / args.steps
-    value = n * args.steps / elapsed
 
-    # Spread: REPEAT_BLOCKS more blocks of graph-launched steps after the timed
-    # region, each bracketed by synchronisation (not part of `value`).
-    block_steps = max(1, min(REPEAT_BLOCK_STEPS, args.steps))
-    blocks = []
-    for _ in range(REPEAT_BLOCKS):
-        t1 = time.perf_counter()
-        sim.step(block_steps)
+    # Pre-collapse figure (steps warmup+1 .. warmup+steps from the lattice),
+    # at the tuning in force; not `value`.
+    early = None
+    if not args.no_early and args.settle > 0:
+        sim.step(args.warmup)
+        sim.prepare()
         sim.synchronize()
-        blocks.append(1e3 * (time.perf_counter() - t1) / block_steps)
-    repeats = {"blocks": REPEAT_BLOCKS, "steps_per_block": block_steps, "median_ms_per_step": float(np.median(blocks)),
-               "min_ms_per_step": min(blocks), "max_ms_per_step": max(blocks)}
+        t0 = time.perf_counter()
+        sim.step(args.steps)
+        sim.synchronize()
+        el = time.perf_counter() - t0
+        early = {"first_step": args.warmup + 1, "steps": args.steps, "ms_per_step": 1e3 * el / args.steps,
+                 "value": n * args.steps / el, "tuning": sim.tuning().as_dict()}
+        for i, sc in enumerate(scenes):
+            sim.init_lattice(pkg.sph.make_lattice(**sc.lattice_args()), scene=i)
 
+    # The workload's state: the dam after `settle` steps, then `warmup` steps.
+    sim.step(args.settle + args.warmup)
+    sim.synchronize()
+    first_step = args.settle + args.warmup + 1
+    if args.tune == "auto":
+        sim.autotune(trial_steps=args.tune_steps)  # restores the state it started from
+    tuning = sim.tuning().as_dict()
+    sim.prepare()  # both step graphs of the chosen tuning, outside the timed region
+    sim.synchronize()
+    region_start = sim.download()
+    warm0 = region_start[:scenes[0].count].copy() if args.cpu_budget > 0 else None
+    progress(f"tuning {tuning}; timing {args.reps} x {args.steps} steps from step {first_step}")
+    elapsed = timed_repetitions(sim, region_start, args.steps, args.reps)
+    med = float(np.median(elapsed))
+    ms_per_step = 1e3 * med / args.steps
+    value = n * args.steps / med
+    reps = {"repetitions": args.reps, "steps_each": args.steps,
+            "ms_per_step_each": [1e3 * e / args.steps for e in elapsed],
+            "median_ms_per_step": ms_per_step, "min_ms_per_step": 1e3 * min(elapsed) / args.steps,
+            "max_ms_per_step": 1e3 * max(elapsed) / args.steps}
     m = sim.metrics(0)
 
     # Per-kernel durations over the timed region itself: its starting state is
     # uploaded again and its steps replayed as ungraphed launches, each kernel
-    # bracketed by HIP events on the solver's stream.  (Within-cell order after
-    # the upload follows the original index, so the replay matches the timed
-    # trajectory statistically, not bit for bit.)
+    # bracketed by HIP events on the solver's stream (the same trajectory as
+    # the timed repetitions, bit for bit).
     timing_steps = args.steps if args.timing_steps is None else min(args.timing_steps, args.steps)
     progress(f"replaying {timing_steps} steps of the timed region with kernel events")
     sim.upload(region_start)
-    del region_start
     sim.set_timing(True)
     sim.step(timing_steps)
     times = sim.kernel_times()
@@ -409,30 +467,31 @@ def main(argv=None):
     roofline = None
     if algo is not None:
         achieved = algo * n / (dom_ms * 1e-3) / 1e9
-        prof = profile_record(args.workload, dom, tuning)
+        prof = profile_record(args.workload, dom, tuning, args.settle)
         traffic = None if prof is None else prof["traffic_bytes_per_launch"]
         roofline = {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                     "traffic_gbs": None if traffic is None else traffic / (dom_ms * 1e-3) / 1e9,
-                    "valu_issue_estimate": None if prof is None else prof.get("valu_issue_estimate"),
                     "algo_bytes_per_particle": algo, "launch_ms": dom_ms,
-                    # traffic and the VALU estimate come from the committed counter
-                    # runs, whose own launches took profile_launch_ms
+                    # traffic and the counter-derived utilisations come from the
+                    # committed counter runs of this workload, state and tuning
                     "profile": None if prof is None else prof.get("source"),
                     "profile_launch_ms": None if prof is None else prof.get("rocprof_avg_ms_timed_launches"),
-                    "binding_roof": "VALU issue: the neighbour gathers are instruction-bound (about 10 VALU "
-                                    "instructions per candidate in the density scan, 12 in the force scan, "
-                                    "23 per neighbour in the force terms), with HBM mostly idle"}
+                    "valu_busy": None if prof is None else prof.get("valu_busy"),
+                    "valu_lane_utilization": None if prof is None else prof.get("valu_lane_utilization"),
+                    "useful_fp32_tflops": None if prof is None else prof.get("useful_fp32_tflops"),
+                    "binding_roof": "VALU issue: the neighbour gathers are instruction-bound (candidate scans and "
+                                    "pair terms on the vector ALUs), with HBM mostly idle"}
     # Host readback of the whole AoS (sph_download into pageable numpy memory):
     # what a frame would cost a caller that takes every step's particles to the
     # host instead of binding the device buffer.  Untimed region; not `value`.
     host = np.empty(n, dtype=pkg.sph.PARTICLE_DTYPE)
     sim.download_into(host)
-    reps = 5
+    nrd = 5
     t0 = time.perf_counter()
-    for _ in range(reps):
+    for _ in range(nrd):
         sim.download_into(host)
-    dl_ms = 1e3 * (time.perf_counter() - t0) / reps
+    dl_ms = 1e3 * (time.perf_counter() - t0) / nrd
     pcie = {"download_ms_per_frame": dl_ms, "bytes": n * pkg.sph.PARTICLE_DTYPE.itemsize,
             "gbs": n * pkg.sph.PARTICLE_DTYPE.itemsize / (dl_ms * 1e-3) / 1e9,
             "value_with_download_every_step": n / ((ms_per_step + dl_ms) * 1e-3)}
@@ -443,17 +502,19 @@ def main(argv=None):
              "peak": HBM_PEAK_GBS, "unit": "GB/s"}
     if build["achieved"] is not None:
         build["frac"] = build["achieved"] / HBM_PEAK_GBS
+    sim.upload(region_start)
     passes = three_passes(sim, n)
     sim.close()
 
     progress(f"plain: {ms_per_step:.3f} ms/step")
     learned = None
     if not args.no_learned and len(scenes) == 1:
-        learned = learned_form(pkg, scenes, args.warmup, args.steps, ms_per_step)
+        learned = learned_form(pkg, scenes, region_start, first_step, args.steps, ms_per_step)
+    del region_start
 
     cpu = None
     if args.cpu_budget > 0:
-        cpu = cpu_baseline(pkg, scenes[0], args.cpu_budget, state=warm0, state_step=args.warmup)
+        cpu = cpu_baseline(pkg, scenes[0], args.cpu_budget, state=warm0, state_step=first_step - 1)
         if len(scenes) > 1:
             cpu["sample"] += " (first scene of the batch)"
 
@@ -465,17 +526,20 @@ def main(argv=None):
         "steps": args.steps,
         "warmup": args.warmup,
         "ms_per_step": ms_per_step,
-        "repeats": repeats,
+        "timed": reps,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": args.workload, "particles": n, "scenes": len(scenes), "form": "plain",
-                   "dt": 1e-4, "spacing": "0.5h jittered lattice", "steps_from_initial_state": args.warmup + 1,
-                   "tuning": tuning},
+                   "state": f"dam-break after {args.settle} settling steps + {args.warmup} warmup steps from the "
+                            "seeded lattice (collapsed phase)",
+                   "dt": 1e-4, "spacing": "0.5h jittered lattice", "settle_steps": args.settle,
+                   "steps_from_initial_state": first_step, "tuning": tuning},
         "step_hbm_frac": STEP_BYTES * value / (HBM_PEAK_GBS * 1e9),
         "roofline": roofline,
+        "early_phase": early,
         "build_roofline": build,
         "three_passes": passes,
         "host_readback": pcie,

@@ -192,7 +192,12 @@ int sph_integrate(sph_ctx* ctx);
  * buffer afterwards equals what the three passes would leave, bit for bit.
  * Steps that a later step of the same call follows may skip the AoS write
  * and work on the sorted state alone; the buffer is complete once the call's
- * work on the stream has finished. */
+ * work on the stream has finished.
+ * nsteps = 0 advances nothing: it prepares the sorted state and captures the
+ * step graphs (16-step and 1-step), so that a later call launches only.
+ * At most 8 graph launches (16 steps each, or 1 for the remainder) are queued
+ * on the stream at a time: a call for more steps waits on the host for the
+ * oldest queued launch before it adds the next. */
 int sph_step(sph_ctx* ctx, int nsteps);
 
 /* Device pointer of the AoS buffer (original index order, stride 64 bytes, pos
@@ -275,8 +280,11 @@ typedef struct sph_train_result {
 typedef struct sph_coarse_opts {
     int32_t use_network;  /* 1: learned messages; 0: the SPH force held over the 4 fine steps */
     int32_t judge_every;  /* every judge_every-th coarse step is also run as 4 plain steps (0: never) */
-    float drift_limit;    /* judge: |mean rho(coarse) - mean rho(plain)| above this keeps the plain result */
-    int32_t cooldown;     /* after a rejected step, this many coarse steps run as 4 plain steps each */
+    float drift_limit;    /* judge: relative drift |mean rho(coarse) - mean rho(plain)| / mean rho(plain)
+                             from the same state; above this the plain result is kept (0.01 = 1%) */
+    int32_t cooldown;     /* after a rejected step, this many coarse steps run as 4 plain steps each,
+                             and the next step is judged again (another rejection restarts the
+                             cooldown); 0: only the judged step is replaced */
 } sph_coarse_opts_t;
 
 typedef struct sph_coarse_stats {
@@ -284,7 +292,7 @@ typedef struct sph_coarse_stats {
     int64_t plain_steps;   /* fine steps run by the plain solver (fallback and cooldown) */
     int64_t judged;
     int64_t fallbacks;
-    double last_drift;
+    double last_drift;     /* relative drift of the last judged step */
     double max_drift;
     int64_t pairs;         /* neighbour pairs the network evaluated in the last message pass */
 } sph_coarse_stats_t;

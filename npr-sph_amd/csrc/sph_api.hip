@@ -25,6 +25,10 @@ thread_local std::string g_create_error;
 
 constexpr double kPi = (double)3.141592741f;  // rho_pres_comp.glsl:8
 constexpr int kUnroll = 16;                    // steps per captured graph
+// sph_step keeps at most this many graph launches queued on the stream: the
+// next launch first waits for the one kMaxGraphsInFlight launches before it.
+// That still leaves 7 x 16 steps of work queued ahead of the GPU.
+constexpr int kMaxGraphsInFlight = 8;
 
 struct GridGeom {
     float lo[3];
@@ -115,6 +119,8 @@ struct sph_ctx {
     bool rho_valid = false;  // rho/D2 (A order) == AoS extras through perm
 
     hipGraphExec_t graph[2] = {nullptr, nullptr};  // 1 step, kUnroll steps
+    hipEvent_t inflight[kMaxGraphsInFlight] = {};  // completion of recent graph launches
+    uint64_t graph_launches = 0;                    // graph launches so far (indexes inflight)
 
     MlpWork mlp;                               // learned coarse step
     bool mlp_loaded = false;
@@ -413,6 +419,8 @@ void free_all(sph_ctx* c)
         (void)hipEventDestroy(p.e1);
     }
     for (auto e : c->event_pool) (void)hipEventDestroy(e);
+    for (auto e : c->inflight)
+        if (e) (void)hipEventDestroy(e);
     const MlpWork& w = c->mlp;
     void* bufs[] = {c->d.aos, c->d.PA, c->d.VA, c->d.PB, c->d.VB, c->d.rho, c->d.D2, c->d.perm, c->d.key,
                     c->d.slot, c->d.torig, c->d.tsrc, c->d.tkey, c->d.count, c->d.start, c->d.bsum, c->d.partials,
@@ -467,6 +475,9 @@ int ensure_targets(sph_ctx* c, uint64_t T)
     uint64_t cap = w.target_cap;
     int rc;
     uint64_t c1 = cap, c2 = cap, c3 = cap, c4 = cap, c5 = cap, c6 = cap, c7 = cap;
+    // Until every buffer has grown, the recorded capacity is 0, so a failed
+    // grow (which leaves its buffer null) makes the next call regrow them all.
+    w.target_cap = 0;
     if ((rc = grow(c, &w.cnt, c1, T, 1, "pair counts")) != SPH_OK) return rc;
     if ((rc = grow(c, &w.off, c2, T + 1, 1, "pair offsets")) != SPH_OK) return rc;
     if ((rc = grow(c, &w.tgt, c3, T, 3, "targets")) != SPH_OK) return rc;
@@ -484,6 +495,7 @@ int ensure_pairs(sph_ctx* c, uint64_t P)
     if (P <= w.pair_cap && w.pj) return SPH_OK;
     uint64_t cap[10];
     for (auto& x : cap) x = w.pair_cap;
+    w.pair_cap = 0;  // as in ensure_targets: a failed grow forces a full regrow next time
     int rc;
     if ((rc = grow(c, &w.pj, cap[0], P, 1, "pairs")) != SPH_OK) return rc;
     if ((rc = grow(c, &w.pt, cap[1], P, 1, "pair targets")) != SPH_OK) return rc;
@@ -909,10 +921,15 @@ int sph_step(sph_ctx* c, int nsteps)
 {
     if (!c) return SPH_ERR_INVALID;
     if (nsteps < 0) return fail(c, SPH_ERR_INVALID, "nsteps must be >= 0");
-    if (nsteps == 0) return SPH_OK;
     int rc = set_device(c);
     if (rc != SPH_OK) return rc;
     if ((rc = ensure_b(c)) != SPH_OK) return rc;
+    if (nsteps == 0) {
+        // Preparation only: the sorted state (above) and both step graphs.
+        if (!c->graph[1] && (rc = capture_graph(c, 1, kUnroll)) != SPH_OK) return rc;
+        if (!c->graph[0] && (rc = capture_graph(c, 0, 1)) != SPH_OK) return rc;
+        return SPH_OK;
+    }
     auto run_steps = [&]() -> int {
         int r;
         if (c->timing) {
@@ -923,8 +940,17 @@ int sph_step(sph_ctx* c, int nsteps)
         const int big = nsteps / kUnroll, small = nsteps % kUnroll;
         if (big && !c->graph[1] && (r = capture_graph(c, 1, kUnroll)) != SPH_OK) return r;
         if (small && !c->graph[0] && (r = capture_graph(c, 0, 1)) != SPH_OK) return r;
-        for (int k = 0; k < big; ++k) SPH_HIP(c, hipGraphLaunch(c->graph[1], c->stream));
-        for (int k = 0; k < small; ++k) SPH_HIP(c, hipGraphLaunch(c->graph[0], c->stream));
+        // Launch q of the context waits for launch q - kMaxGraphsInFlight
+        // (possibly from an earlier call) to finish.
+        for (int k = 0; k < big + small; ++k) {
+            const uint64_t q = c->graph_launches;
+            hipEvent_t& ev = c->inflight[q % kMaxGraphsInFlight];
+            if (!ev) SPH_HIP(c, hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+            if (q >= (uint64_t)kMaxGraphsInFlight) SPH_HIP(c, hipEventSynchronize(ev));
+            SPH_HIP(c, hipGraphLaunch(c->graph[k < big ? 1 : 0], c->stream));
+            SPH_HIP(c, hipEventRecord(ev, c->stream));
+            c->graph_launches = q + 1;
+        }
         return SPH_OK;
     };
     if ((rc = run_steps()) != SPH_OK) {
@@ -1060,14 +1086,26 @@ int sph_autotune(sph_ctx* c, int trial_steps, sph_tuning_t* best, double* best_m
         return fail(c, SPH_ERR_NOMEM, "cannot allocate the autotune snapshot");
     }
     SPH_HIP(c, hipMemcpyAsync(c->d_backup, c->d.aos, sizeof(sph_particle_t) * c->n, hipMemcpyDeviceToDevice, c->stream));
-    const bool was_timing = c->timing;
-    c->timing = false;
     const sph_tuning_t start_tuning = c->d.tuning;
-    hipEvent_t e0 = nullptr, e1 = nullptr;
-    SPH_HIP(c, hipEventCreate(&e0));
-    SPH_HIP(c, hipEventCreate(&e1));
-    // Time `steps` steps of tuning t from the snapshot (after one untimed step
-    // that also captures the graphs); per-step milliseconds in *ms.
+    // Kernel timing is off during the trials and restored, with the events
+    // released, on every way out.
+    struct Guard {
+        sph_ctx* c;
+        bool was_timing;
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        ~Guard()
+        {
+            if (e0) (void)hipEventDestroy(e0);
+            if (e1) (void)hipEventDestroy(e1);
+            c->timing = was_timing;
+        }
+    } guard{c, c->timing};
+    c->timing = false;
+    SPH_HIP(c, hipEventCreate(&guard.e0));
+    SPH_HIP(c, hipEventCreate(&guard.e1));
+    hipEvent_t e0 = guard.e0, e1 = guard.e1;
+    // Time `steps` steps of tuning t from the snapshot, after one untimed step
+    // and the capture of both step graphs; per-step milliseconds in *ms.
     auto trial = [&](const sph_tuning_t& t, int steps, float* ms) -> int {
         int r = sph_set_tuning(c, &t);
         if (r != SPH_OK) return r;
@@ -1076,6 +1114,7 @@ int sph_autotune(sph_ctx* c, int trial_steps, sph_tuning_t* best, double* best_m
             return hip_fail(c, hipErrorUnknown, "restore snapshot");
         invalidate_state(c);
         if ((r = sph_step(c, 1)) != SPH_OK) return r;
+        if ((r = sph_step(c, 0)) != SPH_OK) return r;
         if (hipEventRecord(e0, c->stream) != hipSuccess) return hip_fail(c, hipErrorUnknown, "event");
         if ((r = sph_step(c, steps)) != SPH_OK) return r;
         float total = 0.f;
@@ -1129,9 +1168,6 @@ int sph_autotune(sph_ctx* c, int trial_steps, sph_tuning_t* best, double* best_m
         win = start_tuning;
         win_ms = start_ms;
     }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    c->timing = was_timing;
     const int trial_rc = rc;
     const std::string trial_err = c->err;
     rc = sph_set_tuning(c, trial_rc == SPH_OK ? &win : &start_tuning);
@@ -1205,6 +1241,13 @@ int sph_mlp_train(sph_ctx* c, const sph_ctx* later, const sph_train_opts_t* o, f
     if (o->stride < 1 || o->first >= c->n) return fail(c, SPH_ERR_INVALID, "bad target range");
     int rc = set_device(c);
     if (rc != SPH_OK) return rc;
+    // Training launches are never event-timed; the caller's setting is
+    // restored on every way out.
+    struct TimingOff {
+        sph_ctx* c;
+        bool was;
+        ~TimingOff() { c->timing = was; }
+    } timing_off{c, c->timing};
     c->timing = false;
     SPH_HIP(c, hipStreamSynchronize(later->stream));
     if ((rc = density_pass(c, true)) != SPH_OK) return rc;
@@ -1350,15 +1393,22 @@ int sph_coarse_step(sph_ctx* c, int n, const sph_coarse_opts_t* o, sph_coarse_st
     sph_coarse_stats_t local{};
     sph_coarse_stats_t& s = st ? *st : local;
     s = sph_coarse_stats_t{};
+    // After a rejected judge, `cooldown` coarse steps run as 4 plain steps
+    // each and the step after them is judged again; a rejected re-judge starts
+    // another cooldown, so while the network stays out of tolerance the
+    // trajectory stays plain.
     int cooldown = 0;
+    bool rejudge = false;
     for (int k = 0; k < n; ++k) {
         if (cooldown > 0) {
             --cooldown;
             if ((rc = sph_step(c, 4)) != SPH_OK) return rc;
             s.plain_steps += 4;
+            rejudge = cooldown == 0;
             continue;
         }
-        const bool judge = o->judge_every > 0 && (k % o->judge_every) == 0;
+        const bool judge = rejudge || (o->judge_every > 0 && (k % o->judge_every) == 0);
+        rejudge = false;
         if (!judge) {
             if ((rc = coarse_once(c, o->use_network != 0)) != SPH_OK) return rc;
             s.coarse_steps += 1;
@@ -1377,7 +1427,8 @@ int sph_coarse_step(sph_ctx* c, int n, const sph_coarse_opts_t* o, sph_coarse_st
         if ((rc = coarse_once(c, o->use_network != 0)) != SPH_OK) return rc;
         if ((rc = density_pass(c, true)) != SPH_OK) return rc;
         if ((rc = mean_rho(c, &mc)) != SPH_OK) return rc;
-        const double drift = std::fabs(mc - mp);
+        // Relative drift of the mean density against the plain result.
+        const double drift = std::fabs(mc - mp) / std::fabs(mp);
         s.judged += 1;
         s.last_drift = drift;
         s.max_drift = std::max(s.max_drift, drift);

@@ -346,6 +346,11 @@ class Simulation:
     def step(self, nsteps: int = 1):
         self._check(self._L.sph_step(self._h, int(nsteps)))
 
+    def prepare(self):
+        """sph_step(ctx, 0): build the sorted state and capture the step graphs
+        without advancing, so that the next step() call only launches."""
+        self._check(self._L.sph_step(self._h, 0))
+
     def synchronize(self):
         self._check(self._L.sph_synchronize(self._h))
 
@@ -427,7 +432,7 @@ class Simulation:
         self._check(self._L.sph_mlp_predict(self._h, int(path), out.ctypes.data))
         return out
 
-    def coarse_step(self, n=1, use_network=True, judge_every=0, drift_limit=10.0, cooldown=0) -> CoarseStats:
+    def coarse_step(self, n=1, use_network=True, judge_every=0, drift_limit=0.01, cooldown=0) -> CoarseStats:
         o = CoarseOpts(1 if use_network else 0, judge_every, drift_limit, cooldown)
         st = CoarseStats()
         self._check(self._L.sph_coarse_step(self._h, int(n), C.byref(o), C.byref(st)))

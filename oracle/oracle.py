@@ -122,6 +122,8 @@ def lib():
         L.orc_forces_bf.restype = None
         L.orc_forces_grid.argtypes = [C.c_uint64, _D, _D, _D, _D, cp, fp, bp, _D, _D]
         L.orc_forces_grid.restype = C.c_int
+        L.orc_count_pairs_grid.argtypes = [C.c_uint64, _D, cp, fp, bp, C.c_double, C.POINTER(C.c_uint64)]
+        L.orc_count_pairs_grid.restype = C.c_uint64
         L.orc_integrate.argtypes = [C.c_uint64, _D, _D, _D, _D, bp, fp, C.c_double, _U8]
         L.orc_integrate.restype = None
         L.orc_step.argtypes = [C.c_uint64, _D, _D, _D, _D, _D, cp, bp, fp, C.c_int]
@@ -216,6 +218,19 @@ def forces(x, v, rho, p, c=None, f=None, b=None, grid: bool = True):
     else:
         lib().orc_forces_bf(n, x, v, rho, p, C.byref(_c(c)), C.byref(_f(f)), out, scale)
     return out.reshape(n, 3), scale
+
+
+def count_pairs(x, c=None, f=None, b=None, margin: float = 1e-6):
+    """Ordered neighbour pairs (j != i, r < h) in FP64 over the cell stencil
+    (force_comp.glsl:50-57), and how many of them lie within `margin` of
+    r^2 = h^2, where float32 rounding may decide the side."""
+    x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1)
+    amb = C.c_uint64()
+    total = lib().orc_count_pairs_grid(x.size // 3, x, C.byref(_c(c)), C.byref(_f(f)), C.byref(_b(b)), float(margin),
+                                       C.byref(amb))
+    if total == 2**64 - 1:
+        raise ValueError("invalid grid geometry")
+    return int(total), int(amb.value)
 
 
 def integrate(x, v, force, rho, b=None, f=None, wall_eps: float = 0.0):

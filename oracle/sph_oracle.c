@@ -249,6 +249,34 @@ int orc_density_grid(uint64_t n, const double* x, const orc_constants* c, const 
     return 0;
 }
 
+/* Ordered neighbour pairs (j != i, r < h) over the cell stencil, the pairs
+ * force_comp.glsl:50-57 sums over.  r is taken in FP64; *ambiguous counts the
+ * pairs with |r^2/h^2 - 1| < margin, whose side of r = h float32 rounding may
+ * decide.  Returns the pair count, or UINT64_MAX for an invalid grid. */
+uint64_t orc_count_pairs_grid(uint64_t n, const double* x, const orc_constants* c, const orc_fixed* f,
+                              const orc_boundary* b, double margin, uint64_t* ambiguous)
+{
+    const float hf = orc_smoothing_length(c, f);
+    const double h = (double)hf, h2 = h * h;
+    orc_cells cl;
+    if (orc_cells_build(&cl, n, x, hf, b) != 0) return UINT64_MAX;
+    uint64_t total = 0, amb = 0;
+#pragma omp parallel for schedule(dynamic, 256) reduction(+ : total, amb)
+    for (int64_t i = 0; i < (int64_t)n; ++i) {
+        ORC_FOR_NEIGHBOURS(&cl, i, j, {
+            if (j != (uint64_t)i) {
+                double d0 = x[3 * i] - x[3 * j], d1 = x[3 * i + 1] - x[3 * j + 1], d2 = x[3 * i + 2] - x[3 * j + 2];
+                double r2 = d0 * d0 + d1 * d1 + d2 * d2;
+                if (sqrt(r2) < h) ++total;
+                if (fabs(r2 / h2 - 1.0) < margin) ++amb;
+            }
+        });
+    }
+    orc_cells_free(&cl);
+    if (ambiguous) *ambiguous = amb;
+    return total;
+}
+
 /* force_comp.glsl:48-61 body for one pair.  Accumulates the pressure and
  * viscosity terms and the sum of their magnitudes (for error bounds). */
 static inline void orc_pair_force(uint64_t i, uint64_t j, const double* x, const double* v,

@@ -31,19 +31,27 @@ def test_bad_step_counts_are_refused_before_any_gpu_work():
         assert ">=" in r.stderr, args
 
 
-def test_committed_profiles_match_the_default_tuning():
-    """The bench line's roofline.traffic and VALU estimate come from these
-    records; they apply only when the run's tuning equals the recorded one, and
-    the autotuner keeps the default unless another setting is clearly faster."""
+def test_committed_profile_records_are_keyed_and_sourced():
+    """The bench line's roofline.traffic and counter-derived utilisations come
+    from profiles/traffic_<workload>.json; a record applies only to the
+    kernel, settled state and tuning it was measured at, and names the
+    committed rocprofv3 summary it was computed from."""
+    import glob
     bench = _bench()
-    default = {"cells_per_h": 2, "prune": 1, "block": 128, "blocks_per_xcd": 0}
-    for workload in ("dam1m", "dam8m"):
-        rec = bench.profile_record(workload, "force_integrate", default)
-        assert rec is not None, workload
+    recs = sorted(glob.glob(os.path.join(ROOT, "profiles", "traffic_*.json")))
+    for path in recs:
+        rec = json.load(open(path))
+        for key in ("kernel", "workload", "settle", "tuning", "traffic_bytes_per_launch", "source",
+                    "rocprof_avg_ms_timed_launches", "valu_busy"):
+            assert key in rec, (path, key)
+        assert os.path.basename(path) == f"traffic_{rec['workload']}.json"
         assert rec["traffic_bytes_per_launch"] > 0
-        assert 0.0 < rec["valu_issue_estimate"] < 2.0
+        assert 0.0 < rec["valu_busy"] <= 1.0
         assert os.path.exists(os.path.join(ROOT, rec["source"]))
-    assert bench.profile_record("dam1m", "force_integrate", dict(default, block=256)) is None
+        assert bench.profile_record(rec["workload"], rec["kernel"], rec["tuning"], rec["settle"]) == rec
+        other = dict(rec["tuning"], block=384)
+        assert bench.profile_record(rec["workload"], rec["kernel"], other, rec["settle"]) is None
+        assert bench.profile_record(rec["workload"], rec["kernel"], rec["tuning"], rec["settle"] + 1) is None
 
 
 def test_byte_models_add_up():
@@ -51,5 +59,3 @@ def test_byte_models_add_up():
     assert bench.STEP_BYTES == sum(bench.ALGO_BYTES[k] for k in ("density", "force", "integrate")) == 128
     assert bench.ALGO_BYTES["force_integrate"] == 68
     assert bench.MLP_FLOPS_PER_PAIR == 9728
-    with open(os.path.join(ROOT, "profiles", "traffic_dam1m.json")) as f:
-        assert json.load(f)["tuning"]["block"] == 128

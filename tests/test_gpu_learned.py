@@ -321,3 +321,78 @@ def test_coarse_step_counts_every_pair(pkg, orc, gpu_lib):
         margin = min(margin, float(np.min(np.abs(r2.astype(np.float64) / float(h2) - 1.0))))
     assert margin > 1e-6  # no pair so close to r = h that float32 rounding could decide it
     assert stats.pairs == total and total > 0
+
+
+def test_repeated_rejections_keep_the_plain_trajectory(pkg, orc, ref, gpu_lib):
+    """With a zero drift limit every judge rejects: after each rejection the
+    cooldown runs plain steps and the next step is judged again, so no coarse
+    step is ever accepted and the state equals the plain run bit for bit."""
+    sim, sc = _scene(pkg, orc, n=2000, steps=10)
+    sim.mlp_set(ref.init_params(23), _norm(pkg))
+    s0 = sim.download()
+    st = sim.coarse_step(10, use_network=True, judge_every=50, drift_limit=0.0, cooldown=2)
+    # judged at 0, 3, 6, 9 (each after a 2-step cooldown)
+    assert (st.judged, st.fallbacks, st.coarse_steps, st.plain_steps) == (4, 4, 0, 40)
+    got = sim.download()
+    with pkg.Simulation(sim.n, boundaries=pkg.sph.make_boundary(sc.lower, sc.upper)) as plain:
+        plain.upload(s0)
+        plain.step(40)
+        want = plain.download()
+    assert got["pos"].tobytes() == want["pos"].tobytes() and got["vel"].tobytes() == want["vel"].tobytes()
+    # A limit no drift reaches accepts every judged step.
+    sim.upload(s0)
+    st = sim.coarse_step(3, use_network=True, judge_every=1, drift_limit=1e30, cooldown=2)
+    assert (st.judged, st.fallbacks, st.coarse_steps, st.plain_steps) == (3, 0, 3, 0)
+    sim.close()
+
+
+def test_dam8m_learned_leg(pkg, orc, gpu_lib):
+    """The large mixed run's learned leg at its real size: dam8m (8,000,000
+    particles) after 150 steps, with the committed weights.
+    * The BF16 MFMA messages (the kernel fused into the cell-sorted gather)
+      equal the FP32 message path to BF16 accuracy over all particles.
+    * The pairs one message pass evaluates equal the FP64 grid count of
+      neighbour pairs (r < h, j != i) of that state, up to pairs within 1e-6
+      of r = h, whose side float32 rounding decides.
+    * Three coarse steps judged every step by 4 plain steps: each judged step
+      is accepted or replaced, the drift is a finite relative figure and the
+      state stays finite except the reference's own coincident-pair NaNs."""
+    import sys
+    sys.path.insert(0, os.path.join(ROOT, "npr-sph_amd"))
+    import train_surrogate
+    params, norm, _ = train_surrogate.load_weights()
+    sc = pkg.scenes.dam8m()
+    b = pkg.sph.make_boundary(sc.lower, sc.upper)
+    ob = orc.make_boundary(sc.lower, sc.upper)
+    with pkg.Simulation(sc.count, boundaries=b) as sim:
+        sim.init_lattice(pkg.sph.make_lattice(**sc.lattice_args()))
+        sim.step(150)
+        sim.mlp_set(params, norm)
+        s0 = sim.download()
+        p32 = sim.mlp_predict(path=0)
+        pbf = sim.mlp_predict(path=1)
+        ok = np.all(np.isfinite(p32), axis=1)
+        assert ok.mean() > 0.999
+        assert np.array_equal(ok, np.all(np.isfinite(pbf), axis=1))
+        rel = np.linalg.norm(pbf[ok] - p32[ok]) / np.linalg.norm(p32[ok])
+        assert rel < 2e-2, rel
+        del p32, pbf
+        sim.upload(s0)
+        st = sim.coarse_step(1, use_network=True, judge_every=0, drift_limit=0.0, cooldown=0)
+        want, amb = orc.count_pairs(orc.pos64(s0), b=ob, margin=1e-6)
+        assert want > 100 * sc.count
+        assert abs(st.pairs - want) <= amb, (st.pairs, want, amb)
+        sim.upload(s0)
+        st = sim.coarse_step(3, use_network=True, judge_every=1, drift_limit=0.01, cooldown=0)
+        assert st.judged == 3 and st.coarse_steps + st.fallbacks == 3
+        assert st.plain_steps == 4 * st.fallbacks
+        assert np.isfinite(st.max_drift) and 0.0 <= st.max_drift < 1.0
+        m = sim.metrics(0)
+        assert np.isfinite(m.mean_rho) and m.mean_rho > 1000.0
+        # Non-finite particles: the reference's coincident-pair NaNs (normalize(0)
+        # where corner walls clamp particles onto one point), as in 12 plain steps.
+        sim.upload(s0)
+        sim.step(12)
+        mp = sim.metrics(0)
+        assert mp.nonfinite < 1e-3 * sc.count
+        assert m.nonfinite <= 2 * mp.nonfinite + 100, (m.nonfinite, mp.nonfinite)

@@ -151,11 +151,20 @@ def test_grid_bit_exact(pkg, orc, gpu_lib):
             assert_same_grid(start, perm, cstart, cperm, [g.ncells])
 
 
-def test_fused_step_equals_passes(pkg, orc, gpu_lib):
+TUNINGS = [(sub, prune, block, bpx) for sub in (1, 2, 3) for prune in (0, 1) for block in (128, 256)
+           for bpx in (0, 3)]
+
+
+@pytest.mark.parametrize("tuning", TUNINGS, ids=lambda t: "%d-%d-%d-%d" % t)
+def test_fused_step_equals_passes(pkg, orc, gpu_lib, tuning):
+    """sph_step (the fused force+integrate kernel k_force<true,...> in captured
+    graphs) equals 37 x (rho_pres, force, integrate) bit for bit at every tuning."""
     sc = pkg.scenes.dam10k()
     p0 = _lattice(orc, sc)
     b = pkg.sph.make_boundary(sc.lower, sc.upper)
     with pkg.Simulation(sc.count, boundaries=b) as a, pkg.Simulation(sc.count, boundaries=b) as c:
+        a.set_tuning(*tuning)
+        c.set_tuning(*tuning)
         a.upload(p0)
         c.upload(p0)
         a.step(37)  # 2 x 16-step graph + 5 single-step graphs
@@ -164,6 +173,53 @@ def test_fused_step_equals_passes(pkg, orc, gpu_lib):
             c.force()
             c.integrate()
         assert a.download().tobytes() == c.download().tobytes()
+
+
+def test_fused_step_every_tuning_dam1m_settled(pkg, orc, gpu_lib):
+    """The timed path on the timed state: dam1m after 400 steps (collapsed, about
+    150 neighbours per particle).  From that state, 17 fused steps (one 16-step
+    graph and one 1-step graph) at every tuning give the same bits as the
+    cells_per_h setting's pruned 128-wide one-tile-per-workgroup form; 17 steps
+    of the three-pass surface equal them at the autotuner's usual picks
+    (1,0,128,0) and (2,1,128,0); and one fused step at each cells_per_h, and at
+    (1,0,128,0), matches the FP64 grid transcription pass by pass."""
+    sc = pkg.scenes.dam1m()
+    b = pkg.sph.make_boundary(sc.lower, sc.upper)
+    ob = orc.make_boundary(sc.lower, sc.upper)
+    with pkg.Simulation(sc.count, boundaries=b) as sim:
+        sim.init_lattice(pkg.sph.make_lattice(**sc.lattice_args()))
+        sim.step(400)
+        st = sim.download()
+        ref = {}
+        for t in [(s, 1, 128, 0) for s in (1, 2, 3)] + [x for x in TUNINGS if x[1:] != (1, 128, 0)]:
+            sim.set_tuning(*t)
+            sim.upload(st)
+            sim.step(17)
+            got = sim.download().tobytes()
+            if t[0] not in ref:
+                ref[t[0]] = got
+            else:
+                assert got == ref[t[0]], t
+        for t in [(1, 0, 128, 0), (2, 1, 128, 0)]:
+            sim.set_tuning(*t)
+            sim.upload(st)
+            for _ in range(17):
+                sim.rho_pres()
+                sim.force()
+                sim.integrate()
+            assert sim.download().tobytes() == ref[t[0]], t
+        for t in [(1, 0, 128, 0), (2, 1, 128, 0), (3, 1, 256, 3)]:
+            sim.set_tuning(*t)
+            sim.upload(st)
+            sim.step(1)
+            after = sim.download()
+            check_density(orc, st, after, ob, grid=True)
+            s1 = st.copy()
+            s1["extras"][:, :2] = after["extras"][:, :2]
+            check_force(orc, s1, after, ob, grid=True)
+            s2 = s1.copy()
+            s2["force"][:, :3] = after["force"][:, :3]
+            check_integrate(orc, s2, after, ob)
 
 
 def test_device_lattice_matches_cpu(pkg, orc, gpu_lib):

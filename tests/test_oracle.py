@@ -150,6 +150,31 @@ def test_grid_equals_bruteforce(orc, seed):
     assert np.all(np.abs(f1 - f2).max(axis=1) <= 1e-12 * s1)
 
 
+def test_count_pairs_equals_bruteforce(orc):
+    """Ordered pairs (j != i, r < h) over the cell stencil equal an O(N^2)
+    numpy count, including particles beyond the walls and a coincident pair;
+    a jitter-free 0.5h lattice interior has 26 neighbours per particle."""
+    p = _random_cloud(orc, 1500, 5)
+    p["pos"][:20, 0] = np.float32(0.6)
+    p["pos"][21, :3] = p["pos"][22, :3]
+    x = orc.pos64(p).reshape(-1, 3)
+    h = float(orc.smoothing_length())
+    d2 = ((x[:, None, :] - x[None, :, :]) ** 2).sum(-1)
+    np.fill_diagonal(d2, np.inf)
+    want = int(np.count_nonzero(np.sqrt(d2) < h))
+    got, amb = orc.count_pairs(x, margin=1e-6)
+    assert got == want and want > 0
+    assert amb == int(np.count_nonzero(np.abs(d2 / (h * h) - 1.0) < 1e-6))
+    s = np.float32(0.5) * orc.smoothing_length()
+    lat = orc.lattice(3, 3, 3, s, (0.0, 0.0, 0.0), 0.0, 1)
+    got, _ = orc.count_pairs(orc.pos64(lat))
+    # 27 sites: pairs within h on the lattice offsets {-1,0,1}^3 minus the origin
+    # and the (+-2,0,0)-type offsets at exactly r = h (strict <) are absent.
+    idx = np.array([(i, j, k) for i in range(3) for j in range(3) for k in range(3)], dtype=float)
+    dd = ((idx[:, None, :] - idx[None, :, :]) ** 2).sum(-1)
+    assert got == int(np.count_nonzero((dd > 0) & (np.sqrt(dd) * 0.5 < 1.0)))
+
+
 def test_permutation_equivariance(orc):
     p = _random_cloud(orc, 800, 3)
     perm = np.random.default_rng(4).permutation(800)

@@ -232,6 +232,17 @@ int sph_load(sph_ctx* ctx, const char* path, uint64_t* frame);
 /* Reductions over scene `scene` of the AoS state (waits). */
 int sph_metrics(sph_ctx* ctx, int scene, sph_metrics_t* out);
 
+/* sph_metrics without any allocation: no counterpart in the reference.  For
+ * callers that cannot allocate (serving loops, harnesses that forbid
+ * hipMalloc): it calls no hipMalloc, hipFree or allocating library routine
+ * and keeps its partial sums in a static device array of the library (1 MiB
+ * per GPU, held until the process ends).  Results equal sph_metrics bit for
+ * bit.  Limit: scenes of at most 4,194,304 particles (16,384 tiles of 256);
+ * larger scenes return SPH_ERR_INVALID.  All calls on one GPU share the array:
+ * calls from contexts on different streams of the same GPU that overlap in
+ * time race on it.  Waits for the context's stream. */
+int sph_metrics_noalloc(sph_ctx* ctx, int scene, sph_metrics_t* out);
+
 /* Gather tuning (no reference counterpart).  sph_autotune times every
  * supported setting for `trial_steps` steps from the current state, re-times
  * the four fastest and the setting in force for 4*trial_steps steps, restores

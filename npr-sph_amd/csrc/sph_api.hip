@@ -11,6 +11,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -86,10 +87,47 @@ struct PendingTimer {
     hipEvent_t e0, e1;
 };
 
+// Scratch of the allocation-free entry point sph_metrics_noalloc: per-tile
+// partial sums of up to kNoAllocTiles tiles (kNoAllocTiles * kTile =
+// 4,194,304 particles per scene), then the 7 combined values.
+constexpr int kNoAllocTiles = 16384;
+constexpr size_t kScratchDoubles = (size_t)kNoAllocTiles * 8u + 8u;
+
+}  // namespace
+
+// Every GPU the library runs on holds its own copy of this array (1 MiB) until
+// the process ends.  Host code never passes the name itself to a kernel or a
+// copy: on the host it is only a stand-in, and the device address comes from
+// hipGetSymbolAddress (resolve_scratch).
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on `device`, which must be the current device;
+// looked up once per device.
+hipError_t resolve_scratch(int device, double** out)
+{
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+    if (!g_scratch_ptr[device]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch_ptr[device] = static_cast<double*>(p);
+    }
+    *out = g_scratch_ptr[device];
+    return hipSuccess;
+}
+
 }  // namespace
 
 struct sph_ctx {
     int device = 0;
+    double* scratch = nullptr;  // device address of `scratch` on `device`
     hipStream_t stream = nullptr;
     bool own_stream = false;
     int nscenes = 0;
@@ -525,6 +563,20 @@ int density_pass(sph_ctx* c, bool write_aos)
 
 int mean_rho(sph_ctx* c, double* out);
 
+// Per-scene metrics from the combined per-tile sums acc[0..6].
+void finish_metrics(const double* acc, uint32_t count, sph_metrics_t* out)
+{
+    const double cnt = (double)count;
+    const double fin = std::max(cnt - acc[6], 1.0);
+    out->mean_rho = acc[0] / cnt;
+    out->kinetic = acc[1];
+    out->com[0] = acc[2] / fin;
+    out->com[1] = acc[3] / fin;
+    out->com[2] = acc[4] / fin;
+    out->max_speed = acc[5];
+    out->nonfinite = (uint64_t)acc[6];
+}
+
 }  // namespace
 
 extern "C" {
@@ -580,6 +632,11 @@ int sph_create_batch(sph_ctx** out, int nscenes, const size_t* counts, const sph
     };
     if (hipGetDevice(&c->device) != hipSuccess) {
         c->err = "no HIP device";
+        (void)hipGetLastError();
+        return bail(SPH_ERR_HIP);
+    }
+    if (const hipError_t e = resolve_scratch(c->device, &c->scratch); e != hipSuccess) {
+        c->err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
         (void)hipGetLastError();
         return bail(SPH_ERR_HIP);
     }
@@ -1027,23 +1084,42 @@ int sph_metrics(sph_ctx* c, int scene, sph_metrics_t* out)
     int rc = set_device(c);
     if (rc != SPH_OK) return rc;
     const int t0 = c->tile_begin[scene], nt = c->tile_end[scene] - t0;
-    SPH_RUN(c, "metrics", launch_metrics(c->stream, c->d, t0, nt));
+    SPH_RUN(c, "metrics", launch_metrics(c->stream, c->d, t0, nt, c->d.partials));
     std::vector<double> part((size_t)nt * 8u);
-    SPH_HIP(c, hipMemcpyAsync(part.data(), c->d.partials + (size_t)t0 * 8u, part.size() * sizeof(double),
-                              hipMemcpyDeviceToHost, c->stream));
+    SPH_HIP(c, hipMemcpyAsync(part.data(), c->d.partials, part.size() * sizeof(double), hipMemcpyDeviceToHost,
+                              c->stream));
     SPH_HIP(c, hipStreamSynchronize(c->stream));
     double acc[7] = {0, 0, 0, 0, 0, 0, 0};
     for (int t = 0; t < nt; ++t)
-        for (int k = 0; k < 7; ++k) acc[k] = (k == 5) ? std::max(acc[k], part[(size_t)t * 8 + k]) : acc[k] + part[(size_t)t * 8 + k];
-    const double cnt = (double)c->count[scene];
-    const double fin = std::max(cnt - acc[6], 1.0);
-    out->mean_rho = acc[0] / cnt;
-    out->kinetic = acc[1];
-    out->com[0] = acc[2] / fin;
-    out->com[1] = acc[3] / fin;
-    out->com[2] = acc[4] / fin;
-    out->max_speed = acc[5];
-    out->nonfinite = (uint64_t)acc[6];
+        for (int k = 0; k < 7; ++k) {
+            const double x = part[(size_t)t * 8 + k];
+            acc[k] = (k == 5) ? (x > acc[k] ? x : acc[k]) : acc[k] + x;
+        }
+    finish_metrics(acc, c->count[scene], out);
+    return SPH_OK;
+}
+
+int sph_metrics_noalloc(sph_ctx* c, int scene, sph_metrics_t* out)
+{
+    if (!c) return SPH_ERR_INVALID;
+    if (!out || scene < 0 || scene >= c->nscenes) return fail(c, SPH_ERR_INVALID, "bad metrics arguments");
+    const int t0 = c->tile_begin[scene], nt = c->tile_end[scene] - t0;
+    if (nt > kNoAllocTiles)
+        return fail(c, SPH_ERR_INVALID,
+                    "sph_metrics_noalloc: scene of " + std::to_string(c->count[scene]) +
+                        " particles exceeds the allocation-free limit of " +
+                        std::to_string((long long)kNoAllocTiles * kTile) + " (use sph_metrics)");
+    int rc = set_device(c);
+    if (rc != SPH_OK) return rc;
+    if (!c->scratch) return fail(c, SPH_ERR_STATE, "allocation-free scratch not resolved for this device");
+    double* partials = c->scratch;
+    double* folded = c->scratch + (size_t)kNoAllocTiles * 8u;
+    SPH_RUN(c, "metrics", launch_metrics(c->stream, c->d, t0, nt, partials));
+    SPH_RUN(c, "metrics_fold", launch_metrics_fold(c->stream, partials, nt, folded));
+    double acc[8];
+    SPH_HIP(c, hipMemcpyAsync(acc, folded, sizeof(double) * 7u, hipMemcpyDeviceToHost, c->stream));
+    SPH_HIP(c, hipStreamSynchronize(c->stream));
+    finish_metrics(acc, c->count[scene], out);
     return SPH_OK;
 }
 

@@ -126,7 +126,10 @@ hipError_t launch_density(hipStream_t s, const DeviceState& d, bool write_aos);
 enum class ForceMode : int { Pass = 0, FusedAos = 1, Fused = 2 };
 hipError_t launch_force(hipStream_t s, const DeviceState& d, ForceMode mode);
 hipError_t launch_integrate_aos(hipStream_t s, const DeviceState& d);
-hipError_t launch_metrics(hipStream_t s, const DeviceState& d, int first_tile, int ntiles);
+// Per-tile partials of tiles [first_tile, first_tile+ntiles) into partials[8*ntiles];
+// the fold combines them into out[7] in tile order.
+hipError_t launch_metrics(hipStream_t s, const DeviceState& d, int first_tile, int ntiles, double* partials);
+hipError_t launch_metrics_fold(hipStream_t s, const double* partials, int ntiles, double* out);
 
 hipError_t launch_pairs(hipStream_t s, const DeviceState& d, int nscenes, const MlpWork& w, uint32_t first,
                         uint32_t stride, uint32_t T, bool emit);

@@ -648,6 +648,7 @@ __global__ __launch_bounds__(kTile) void k_integrate_aos(const Tile* __restrict_
 // Per-tile partial sums over the AoS: rho, kinetic energy, position, max
 // speed, count of non-finite pos/vel (kinetic energy, position and speed over
 // finite particles only).  Eight doubles per tile.
+// Tile first_tile + b writes its partials to partials[8b .. 8b+7].
 __global__ __launch_bounds__(kTile) void k_metrics(const Tile* __restrict__ tiles, const SceneDev* __restrict__ scenes,
                                                    const sph_particle_t* __restrict__ aos, double* __restrict__ partials,
                                                    int first_tile)
@@ -687,8 +688,24 @@ __global__ __launch_bounds__(kTile) void k_metrics(const Tile* __restrict__ tile
         const int k = threadIdx.x;
         double x = red[k][0];
         for (int q = 1; q < kTile / 64; ++q) x = (k == 5) ? fmax(x, red[k][q]) : x + red[k][q];
-        partials[(size_t)ti * 8 + k] = x;
+        partials[(size_t)blockIdx.x * 8 + k] = x;
     }
+}
+
+// out[k] = the ntiles per-tile partials of metric k combined in tile order
+// (sum, or max for k = 5), one lane per metric: the same association as the
+// host-side combination in sph_metrics, so both give the same bits.
+__global__ __launch_bounds__(64) void k_metrics_fold(const double* __restrict__ partials, int ntiles,
+                                                     double* __restrict__ out)
+{
+    const int k = threadIdx.x;
+    if (k >= 7) return;
+    double acc = 0.0;
+    for (int t = 0; t < ntiles; ++t) {
+        const double x = partials[(size_t)t * 8 + k];
+        acc = (k == 5) ? (x > acc ? x : acc) : acc + x;
+    }
+    out[k] = acc;
 }
 
 inline unsigned blocks_for(uint32_t n) { return (n + 255u) / 256u; }
@@ -850,10 +867,16 @@ hipError_t launch_integrate_aos(hipStream_t s, const DeviceState& d)
     return hipGetLastError();
 }
 
-hipError_t launch_metrics(hipStream_t s, const DeviceState& d, int first_tile, int ntiles)
+hipError_t launch_metrics(hipStream_t s, const DeviceState& d, int first_tile, int ntiles, double* partials)
 {
     if (ntiles <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_metrics, dim3(ntiles), dim3(kTile), 0, s, d.tiles, d.scenes, d.aos, d.partials, first_tile);
+    hipLaunchKernelGGL(k_metrics, dim3(ntiles), dim3(kTile), 0, s, d.tiles, d.scenes, d.aos, partials, first_tile);
+    return hipGetLastError();
+}
+
+hipError_t launch_metrics_fold(hipStream_t s, const double* partials, int ntiles, double* out)
+{
+    hipLaunchKernelGGL(k_metrics_fold, dim3(1), dim3(64), 0, s, partials, ntiles, out);
     return hipGetLastError();
 }
 

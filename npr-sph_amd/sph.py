@@ -33,7 +33,7 @@ EXPORTS = [
     "sph_device_buffer", "sph_stream", "sph_synchronize", "sph_grid_info", "sph_total_cells", "sph_download_grid",
     "sph_metrics", "sph_set_timing", "sph_kernel_times", "sph_get_tuning", "sph_set_tuning", "sph_autotune",
     "sph_mlp_set", "sph_mlp_get", "sph_mlp_train", "sph_mlp_predict", "sph_coarse_step", "sph_copy_state",
-    "sph_save", "sph_load",
+    "sph_save", "sph_load", "sph_metrics_noalloc",
 ]
 
 # Snapshot files (sph_save / sph_load): 64-byte header, per-scene counts, records.
@@ -196,6 +196,7 @@ def load_library(path: str | None = None):
         "sph_save": (C.c_int, [vp, C.c_char_p, C.c_uint64]),
         "sph_load": (C.c_int, [vp, C.c_char_p, C.POINTER(C.c_uint64)]),
         "sph_metrics": (C.c_int, [vp, C.c_int, C.POINTER(Metrics)]),
+        "sph_metrics_noalloc": (C.c_int, [vp, C.c_int, C.POINTER(Metrics)]),
         "sph_set_timing": (C.c_int, [vp, C.c_int]),
         "sph_kernel_times": (C.c_int, [vp, C.c_int, vp, vp, vp]),
         "sph_get_tuning": (C.c_int, [vp, C.POINTER(Tuning)]),
@@ -391,6 +392,13 @@ class Simulation:
     def metrics(self, scene: int = 0) -> Metrics:
         m = Metrics()
         self._check(self._L.sph_metrics(self._h, scene, C.byref(m)))
+        return m
+
+    def metrics_noalloc(self, scene: int = 0) -> Metrics:
+        """sph_metrics_noalloc: the same reductions with no allocation (static
+        device scratch; scenes of at most 4,194,304 particles)."""
+        m = Metrics()
+        self._check(self._L.sph_metrics_noalloc(self._h, scene, C.byref(m)))
         return m
 
     def tuning(self) -> Tuning:

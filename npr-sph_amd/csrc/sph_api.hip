@@ -154,7 +154,13 @@ struct sph_ctx {
 
     bool b_valid = false;    // B pos/vel == AoS pos/vel through perm
     bool a_valid = false;    // A pos/vel == AoS through perm and the grid matches A
-    bool rho_valid = false;  // rho/D2 (A order) == AoS extras through perm
+    bool rho_valid = false;  // rho, PA.w, VA.w (A order) == AoS extras through perm
+    // B's cells are counted (count[], key, slot): the next build starts at the
+    // scan.  The fused force+integrate kernel leaves this state.
+    bool counted = false;
+    // count[] may hold counts no build will consume (the state was dropped or
+    // the grid changed after a counting step): cleared before the next count.
+    bool counts_dirty = false;
 
     hipGraphExec_t graph[2] = {nullptr, nullptr};  // 1 step, kUnroll steps
     hipEvent_t inflight[kMaxGraphsInFlight] = {};  // completion of recent graph launches
@@ -273,6 +279,8 @@ void select_gather_tiles(sph_ctx* c)
     }
 }
 
+void drop_counts(sph_ctx* c);
+
 // (Re)compute every scene's grid and parameter block; grow the cell buffers
 // when needed and upload the parameter blocks.
 int apply_params(sph_ctx* c, bool geometry_may_change)
@@ -306,6 +314,7 @@ int apply_params(sph_ctx* c, bool geometry_may_change)
         c->cell_cap = cap;
         hipError_t e = launch_fill_u32(c->stream, c->d.count, cap, 0u);
         if (e != hipSuccess) return hip_fail(c, e, "clear cell counts");
+        c->counted = c->counts_dirty = false;
     }
     const int nb = scan_blocks(cells);
     if (nb > c->bsum_cap) {
@@ -333,6 +342,7 @@ int apply_params(sph_ctx* c, bool geometry_may_change)
     if (changed) {
         destroy_graphs(c);
         c->a_valid = false;
+        drop_counts(c);  // counted with the old grid
     }
     return SPH_OK;
 }
@@ -378,20 +388,43 @@ int run(sph_ctx* c, const char* name, F&& launch)
         if (rc_ != SPH_OK) return rc_;                     \
     } while (0)
 
+// Cell counts of B (valid B required), unless the last fused step left them.
+int ensure_counted(sph_ctx* c)
+{
+    if (c->counted) return SPH_OK;
+    if (c->counts_dirty) {
+        SPH_RUN(c, "clear_counts", launch_fill_u32(c->stream, c->d.count, c->cell_cap, 0u));
+        c->counts_dirty = false;
+    }
+    SPH_RUN(c, "hash_count", launch_hash_count(c->stream, c->d));
+    c->counted = true;
+    return SPH_OK;
+}
+
+// The counts drop out of use: a later count must clear them first.
+void drop_counts(sph_ctx* c)
+{
+    if (c->counted) c->counts_dirty = true;
+    c->counted = false;
+}
+
 int enqueue_build(sph_ctx* c)
 {
     const DeviceState& d = c->d;
-    SPH_RUN(c, "hash_count", launch_hash_count(c->stream, d));
+    int rc = ensure_counted(c);
+    if (rc != SPH_OK) return rc;
     SPH_RUN(c, "scan_reduce", launch_scan_reduce(c->stream, d));
     if (!scan_folds(d.total_cells)) SPH_RUN(c, "scan_top", launch_scan_top(c->stream, d));
-    SPH_RUN(c, "scan_down", launch_scan_down(c->stream, d));
+    SPH_RUN(c, "scan_down", launch_scan_down(c->stream, d));  // also clears count[]
+    c->counted = false;
     SPH_RUN(c, "scatter", launch_scatter(c->stream, d));
     SPH_RUN(c, "rank_reorder", launch_rank_reorder(c->stream, d));
     return SPH_OK;
 }
 
 // One step on the sorted state.  write_aos = false leaves the AoS buffer to a
-// later step of the same call: no kernel of a step reads it.
+// later step of the same call: no kernel of a step reads it.  The fused kernel
+// also counts the new state into the next grid's cells.
 int enqueue_step(sph_ctx* c, bool write_aos)
 {
     int rc = enqueue_build(c);
@@ -399,6 +432,7 @@ int enqueue_step(sph_ctx* c, bool write_aos)
     SPH_RUN(c, "density", launch_density(c->stream, c->d, false));
     SPH_RUN(c, "force_integrate",
             launch_force(c->stream, c->d, write_aos ? ForceMode::FusedAos : ForceMode::Fused));
+    c->counted = true;
     return SPH_OK;
 }
 
@@ -411,13 +445,20 @@ int ensure_b(sph_ctx* c)
     return SPH_OK;
 }
 
+// Captures `steps` steps.  Every graph starts and ends with B counted (the
+// caller counts B before the first launch), so its first build has no hash;
+// capturing launches nothing, so the flags are restored afterwards.
 int capture_graph(sph_ctx* c, int which, int steps)
 {
     hipGraph_t g = nullptr;
+    const bool counted = c->counted, dirty = c->counts_dirty;
+    c->counted = true;
     SPH_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
     int rc = SPH_OK;
     for (int k = 0; k < steps && rc == SPH_OK; ++k) rc = enqueue_step(c, k == steps - 1);
     hipError_t e = hipStreamEndCapture(c->stream, &g);
+    c->counted = counted;
+    c->counts_dirty = dirty;
     if (rc != SPH_OK) {
         if (g) (void)hipGraphDestroy(g);
         return rc;
@@ -435,6 +476,7 @@ int capture_graph(sph_ctx* c, int which, int steps)
 void invalidate_state(sph_ctx* c)
 {
     c->a_valid = c->b_valid = c->rho_valid = false;
+    drop_counts(c);
 }
 
 int check_scene(sph_ctx* c, int scene)
@@ -460,7 +502,7 @@ void free_all(sph_ctx* c)
     for (auto e : c->inflight)
         if (e) (void)hipEventDestroy(e);
     const MlpWork& w = c->mlp;
-    void* bufs[] = {c->d.aos, c->d.PA, c->d.VA, c->d.PB, c->d.VB, c->d.rho, c->d.D2, c->d.perm, c->d.key,
+    void* bufs[] = {c->d.aos, c->d.PA, c->d.VA, c->d.PB, c->d.VB, c->d.rho, c->d.perm, c->d.key,
                     c->d.slot, c->d.torig, c->d.tsrc, c->d.tkey, c->d.count, c->d.start, c->d.bsum, c->d.partials,
                     c->d_scenes, c->d_tiles, c->d_tiles128, c->d_backup, c->d_judge_s, c->d_judge_p, c->d_pred_aos,
                     w.params, w.adam_m, w.adam_v, w.grad, w.cnt, w.off, w.tgt, w.wt, w.pred, w.dpred, w.lossp,
@@ -548,7 +590,7 @@ int ensure_pairs(sph_ctx* c, uint64_t P)
 }
 
 // Density pass on the current AoS state (sph_rho_pres without the public
-// entry point), leaving A, rho/D2 and the grid valid.
+// entry point), leaving A, rho, PA.w/VA.w and the grid valid.
 int density_pass(sph_ctx* c, bool write_aos)
 {
     int rc;
@@ -687,7 +729,7 @@ int sph_create_batch(sph_ctx** out, int nscenes, const size_t* counts, const sph
     bool ok = dalloc(&d.aos, n) == hipSuccess && dalloc(&d.PA, (size_t)n + 1u) == hipSuccess &&
               dalloc(&d.VA, (size_t)n + 1u) == hipSuccess &&
               dalloc(&d.PB, n) == hipSuccess && dalloc(&d.VB, n) == hipSuccess && dalloc(&d.rho, n) == hipSuccess &&
-              dalloc(&d.D2, n) == hipSuccess && dalloc(&d.perm, n) == hipSuccess && dalloc(&d.key, n) == hipSuccess &&
+              dalloc(&d.perm, n) == hipSuccess && dalloc(&d.key, n) == hipSuccess &&
               dalloc(&d.slot, n) == hipSuccess && dalloc(&d.torig, n) == hipSuccess &&
               dalloc(&d.tsrc, n) == hipSuccess && dalloc(&d.tkey, n) == hipSuccess &&
               dalloc(&d.partials, (size_t)d.ntiles * 8u) == hipSuccess &&
@@ -981,8 +1023,9 @@ int sph_step(sph_ctx* c, int nsteps)
     int rc = set_device(c);
     if (rc != SPH_OK) return rc;
     if ((rc = ensure_b(c)) != SPH_OK) return rc;
+    if (!c->timing && (rc = ensure_counted(c)) != SPH_OK) return rc;  // graphs start from counted B
     if (nsteps == 0) {
-        // Preparation only: the sorted state (above) and both step graphs.
+        // Preparation only: the sorted, counted state (above) and both step graphs.
         if (!c->graph[1] && (rc = capture_graph(c, 1, kUnroll)) != SPH_OK) return rc;
         if (!c->graph[0] && (rc = capture_graph(c, 0, 1)) != SPH_OK) return rc;
         return SPH_OK;
@@ -1020,6 +1063,7 @@ int sph_step(sph_ctx* c, int nsteps)
     c->b_valid = true;
     c->a_valid = false;
     c->rho_valid = false;
+    c->counted = true;  // the last fused step counted B
     return SPH_OK;
 }
 

@@ -38,8 +38,7 @@ struct DeviceState {
     float4* VA;
     float4* PB;
     float4* VB;
-    float* rho;       // rho of slot s (A order)
-    float2* D2;       // (p, 1/rho) of slot s
+    float* rho;       // rho of slot s (A order); p and 1/rho are PA.w and VA.w
     uint32_t* perm;   // AoS index of slot s
     uint32_t* key;    // cell key of slot s (B order)
     uint32_t* slot;   // arrival rank of slot s within its cell

@@ -141,12 +141,12 @@ __global__ __launch_bounds__(256) void k_lattice(sph_particle_t* __restrict__ ao
 }
 
 // Copy AoS state into slot order B (pos/vel) and/or A (rho, p) through perm.
-// The extras also go to the w lanes of A: PA.w = p, VA.w = 1/rho.
+// The extras go to rho and the w lanes of A: PA.w = p, VA.w = 1/rho.
 __global__ __launch_bounds__(kTile) void k_refresh(const Tile* __restrict__ tiles, const sph_particle_t* __restrict__ aos,
                                                    const uint32_t* __restrict__ perm, float4* __restrict__ P,
                                                    float4* __restrict__ V, float* __restrict__ rho,
-                                                   float2* __restrict__ D2, float* __restrict__ PAw,
-                                                   float* __restrict__ VAw, int pos_vel, int extras)
+                                                   float* __restrict__ PAw, float* __restrict__ VAw, int pos_vel,
+                                                   int extras)
 {
     const Tile t = tiles[blockIdx.x];
     const uint32_t s = t.begin + threadIdx.x;
@@ -161,7 +161,6 @@ __global__ __launch_bounds__(kTile) void k_refresh(const Tile* __restrict__ tile
         const float4 e = r[3];
         const float inv = 1.0f / e.x;
         rho[s] = e.x;
-        D2[s] = make_float2(e.y, inv);
         PAw[4 * (size_t)s + 3] = e.y;
         VAw[4 * (size_t)s + 3] = inv;
     }
@@ -169,9 +168,30 @@ __global__ __launch_bounds__(kTile) void k_refresh(const Tile* __restrict__ tile
 
 // ------------------------------------------------------- neighbour grid ---
 
-// Cell key of every slot of B, and its arrival rank inside the cell.  Lanes of
-// a wave that hold a run of equal keys (cell-sorted input makes runs the
-// common case) share one atomic.
+// Counts slot s (cell key k) into its cell and stores its arrival rank there.
+// Called by the whole wave; lanes with active = false count nothing.  Lanes
+// holding a run of equal keys (cell-sorted slots make runs the common case)
+// share one atomic.
+__device__ __forceinline__ void count_into_cell(uint32_t k, bool active, uint32_t s, uint32_t* __restrict__ slot,
+                                                uint32_t* __restrict__ count)
+{
+    const int lane = lane_id();
+    const uint32_t prev = __shfl_up(k, 1u);
+    const bool head = !active || lane == 0 || prev != k;
+    const uint64_t heads = __ballot(head);
+    const uint64_t upto = (lane == 63) ? ~0ull : ((2ull << lane) - 1ull);  // bits 0..lane
+    const int my_head = 63 - __clzll((long long)(heads & upto));
+    const uint64_t after = heads & ~upto;
+    const int next = after ? (__ffsll((long long)after) - 1) : kWave;
+    uint32_t base = 0;
+    if (head && active) base = atomicAdd(&count[k], (uint32_t)(next - lane));
+    base = __shfl(base, my_head);
+    if (active) slot[s] = base + (uint32_t)(lane - my_head);
+}
+
+// Cell key of every slot of B, and its arrival rank inside the cell.  The
+// fused force+integrate kernel does the same for the state it writes, so a
+// step that follows one starts its build at the scan.
 __global__ __launch_bounds__(kTile) void k_hash_count(const Tile* __restrict__ tiles, const SceneDev* __restrict__ scenes,
                                                       const float4* __restrict__ P, uint32_t* __restrict__ key,
                                                       uint32_t* __restrict__ slot, uint32_t* __restrict__ count)
@@ -186,18 +206,7 @@ __global__ __launch_bounds__(kTile) void k_hash_count(const Tile* __restrict__ t
         k = global_key(sc, p.x, p.y, p.z);
         key[s] = k;
     }
-    const int lane = lane_id();
-    const uint32_t prev = __shfl_up(k, 1u);
-    const bool head = !active || lane == 0 || prev != k;
-    const uint64_t heads = __ballot(head);
-    const uint64_t upto = (lane == 63) ? ~0ull : ((2ull << lane) - 1ull);  // bits 0..lane
-    const int my_head = 63 - __clzll((long long)(heads & upto));
-    const uint64_t after = heads & ~upto;
-    const int next = after ? (__ffsll((long long)after) - 1) : kWave;
-    uint32_t base = 0;
-    if (head && active) base = atomicAdd(&count[k], (uint32_t)(next - lane));
-    base = __shfl(base, my_head);
-    if (active) slot[s] = base + (uint32_t)(lane - my_head);
+    count_into_cell(k, active, s, slot, count);
 }
 
 __global__ __launch_bounds__(kScanThreads) void k_scan_reduce(const uint32_t* __restrict__ count, uint32_t n,
@@ -362,16 +371,18 @@ __device__ __forceinline__ void for_each_gather_tile(uint32_t ntiles, F&& f)
 
 // rho_pres_comp.glsl:35-58.  rho_i = m*315/(64 pi h^9) * sum_{r<h} (h^2-r^2)^3
 // over the stencil (self included, :46); p_i = max(k(rho_i-rho0), 0).
-// Besides rho and (p, 1/rho) the pass stores p in P[s].w and 1/rho in
-// V[s].w, so that the force pass gets a neighbour's whole record in two
-// 16-byte loads.  Candidates are read through P's xyz lanes only; the w lanes
-// are written through Pw/Vw (the same buffers, as floats), so no lane is both
-// read and written by the pass.
+// Besides rho the pass stores p in P[s].w and 1/rho in V[s].w, so that the
+// force pass (and the learned step) gets a neighbour's whole record in two
+// 16-byte loads.  P[s] is rewritten whole, (x, y, z, p), from the position
+// the lane loaded: a 16-byte store fills its slot instead of leaving a
+// partly written one, and other waves reading that candidate concurrently
+// see the same x, y, z bits before and after.  Candidates are read through
+// the buffer resource of P; V's w lane is written through Vw (V as floats).
 template <int SUB, bool PRUNE, int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_density(const Tile* __restrict__ tiles, const SceneDev* __restrict__ scenes,
-                                                   const uint32_t* __restrict__ start, const float4* __restrict__ P,
-                                                   float* __restrict__ rho_out, float2* __restrict__ D2, float* Pw,
-                                                   float* Vw, const uint32_t* __restrict__ perm,
+                                                   const uint32_t* __restrict__ start, float4* P,
+                                                   float* __restrict__ rho_out, float* Vw,
+                                                   const uint32_t* __restrict__ perm,
                                                    sph_particle_t* __restrict__ aos, uint32_t np, uint32_t ntiles)
 {
     __shared__ uint32_t lds[BLOCK / 64][gather_lds_words<SUB>()];
@@ -410,8 +421,7 @@ __global__ __launch_bounds__(BLOCK) void k_density(const Tile* __restrict__ tile
         const float p = fmaxf(sc.gas_const * (rho - sc.rho0), 0.f);
         const float inv = 1.0f / rho;
         rho_out[s] = rho;
-        D2[s] = make_float2(p, inv);
-        Pw[4 * (size_t)s + 3] = p;
+        P[s] = make_float4(xi.x, xi.y, xi.z, p);
         Vw[4 * (size_t)s + 3] = inv;
         if (aos) {
             float* e = aos[perm[s]].extras;
@@ -443,7 +453,8 @@ static_assert((kQueue & (kQueue - 1)) == 0 && kQueue >= 16, "queue ring size");
 // fills the second entry of a queue pair that a lane does not have.  P and V
 // hold sentinel + 1 elements.
 // FUSE additionally runs integrate_comp.glsl:35-81 on the new force, writes the
-// next step's B state and, unless aos is null, the AoS record (pos, vel,
+// next step's B state, counts it into the next grid's cells (as k_hash_count
+// would from B) and, unless aos is null, writes the AoS record (pos, vel,
 // force, extras.xy).  No kernel of a step reads the AoS, so only the last step
 // of a multi-step call needs to write it.
 // Occupancy: at 1 or 2 cells per h the kernel fits 96 VGPRs (5 waves per SIMD
@@ -454,8 +465,9 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(SUB < 3 ?
                                                  const uint32_t* __restrict__ start, const float4* __restrict__ P,
                                                  const float4* __restrict__ V, const float* __restrict__ rho,
                                                  const uint32_t* __restrict__ perm, sph_particle_t* __restrict__ aos,
-                                                 float4* __restrict__ PN, float4* __restrict__ VN, uint32_t sentinel,
-                                                 uint32_t ntiles)
+                                                 float4* __restrict__ PN, float4* __restrict__ VN,
+                                                 uint32_t* __restrict__ key, uint32_t* __restrict__ slot,
+                                                 uint32_t* __restrict__ count, uint32_t sentinel, uint32_t ntiles)
 {
     __shared__ uint32_t queue[kQueue][BLOCK];
     __shared__ uint32_t lds[BLOCK / 64][gather_lds_words<SUB>()];
@@ -565,7 +577,6 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(SUB < 3 ?
                                     }
                                 });
         flush(true);
-        if (!active) return;
         float f[3];
         f[0] = fmaf(sc.pres_m, ax, fmaf(sc.visc_m, bx, rhoi * sc.grav[0]));
         f[1] = fmaf(sc.pres_m, ay, fmaf(sc.visc_m, by, rhoi * sc.grav[1]));
@@ -574,9 +585,15 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(SUB < 3 ?
             const float x[3] = {xi.x, xi.y, xi.z}, v[3] = {vi.x, vi.y, vi.z};
             float nx[3], nv[3];
             integrate_one(sc, x, v, f, rhoi, nx, nv);
-            PN[s] = make_float4(nx[0], nx[1], nx[2], 0.f);
-            VN[s] = make_float4(nv[0], nv[1], nv[2], 0.f);
-            if (!aos) return;  // a step whose record a later step of the same call overwrites
+            uint32_t k = 0xffffffffu;
+            if (active) {
+                PN[s] = make_float4(nx[0], nx[1], nx[2], 0.f);
+                VN[s] = make_float4(nv[0], nv[1], nv[2], 0.f);
+                k = global_key(sc, nx[0], nx[1], nx[2]);
+                key[s] = k;
+            }
+            count_into_cell(k, active, s, slot, count);  // the whole wave takes part
+            if (!active || !aos) return;  // no record, or one a later step of the same call overwrites
             float* r = reinterpret_cast<float*>(aos + perm[s]);
             r[0] = nx[0];
             r[1] = nx[1];
@@ -590,6 +607,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(SUB < 3 ?
             r[12] = rhoi;
             r[13] = pi;
         } else {
+            if (!active) return;
             float* r = reinterpret_cast<float*>(aos + perm[s]);
             r[8] = f[0];
             r[9] = f[1];
@@ -750,7 +768,7 @@ hipError_t launch_lattice(hipStream_t s, sph_particle_t* aos, uint32_t first, co
 
 hipError_t launch_refresh(hipStream_t s, const DeviceState& d, bool pos_vel, bool extras)
 {
-    hipLaunchKernelGGL(k_refresh, dim3(d.ntiles), dim3(kTile), 0, s, d.tiles, d.aos, d.perm, d.PB, d.VB, d.rho, d.D2,
+    hipLaunchKernelGGL(k_refresh, dim3(d.ntiles), dim3(kTile), 0, s, d.tiles, d.aos, d.perm, d.PB, d.VB, d.rho,
                        reinterpret_cast<float*>(d.PA), reinterpret_cast<float*>(d.VA), pos_vel ? 1 : 0,
                        extras ? 1 : 0);
     return hipGetLastError();
@@ -809,8 +827,8 @@ template <int SUB, bool PRUNE, int BLOCK>
 hipError_t density_variant(hipStream_t s, const DeviceState& d, int write_aos)
 {
     hipLaunchKernelGGL((k_density<SUB, PRUNE, BLOCK>), gather_grid(d), dim3(BLOCK), 0, s, d.gtiles, d.scenes, d.start,
-                       d.PA, d.rho, d.D2, reinterpret_cast<float*>(d.PA), reinterpret_cast<float*>(d.VA), d.perm,
-                       write_aos ? d.aos : nullptr, d.n, (uint32_t)d.ngtiles);
+                       d.PA, d.rho, reinterpret_cast<float*>(d.VA), d.perm, write_aos ? d.aos : nullptr, d.n,
+                       (uint32_t)d.ngtiles);
     return hipGetLastError();
 }
 
@@ -821,10 +839,11 @@ hipError_t force_variant(hipStream_t s, const DeviceState& d, int mode)
     if (mode != (int)ForceMode::Pass)
         hipLaunchKernelGGL((k_force<true, SUB, PRUNE, BLOCK>), gather_grid(d), dim3(BLOCK), 0, s, d.gtiles, d.scenes,
                            d.start, d.PA, d.VA, d.rho, d.perm, mode == (int)ForceMode::FusedAos ? d.aos : nullptr,
-                           d.PB, d.VB, d.n, (uint32_t)d.ngtiles);
+                           d.PB, d.VB, d.key, d.slot, d.count, d.n, (uint32_t)d.ngtiles);
     else
         hipLaunchKernelGGL((k_force<false, SUB, PRUNE, BLOCK>), gather_grid(d), dim3(BLOCK), 0, s, d.gtiles, d.scenes,
-                           d.start, d.PA, d.VA, d.rho, d.perm, d.aos, d.PB, d.VB, d.n, (uint32_t)d.ngtiles);
+                           d.start, d.PA, d.VA, d.rho, d.perm, d.aos, d.PB, d.VB, nullptr, nullptr, nullptr, d.n,
+                           (uint32_t)d.ngtiles);
     return hipGetLastError();
 }
 

@@ -60,8 +60,9 @@ __device__ __forceinline__ uint32_t scene_of(const SceneDev* __restrict__ scenes
 
 // Features of pair (i = slot s, j) in float32 (the pressure term is the
 // force_comp.glsl:59 factor (p_i+p_j)/(2 rho_j)).
+// xj.w holds p_j and vj.w 1/rho_j, as the density pass leaves them.
 __device__ __forceinline__ void pair_features(const SceneDev& sc, const MlpNorm& nm, float4 xi, float4 vi, float rhoi,
-                                              float pi, float4 xj, float4 vj, float rhoj, float2 dj, float f[kNin])
+                                              float pi, float4 xj, float4 vj, float rhoj, float f[kNin])
 {
     const float inv_h = 1.0f / sc.h;
     f[0] = (xi.x - xj.x) * inv_h;
@@ -72,14 +73,14 @@ __device__ __forceinline__ void pair_features(const SceneDev& sc, const MlpNorm&
     f[5] = (vi.z - vj.z) * nm.v_scale;
     f[6] = rhoi * nm.rho_scale;
     f[7] = rhoj * nm.rho_scale;
-    f[8] = (pi + dj.x) * (0.5f * dj.y) * nm.pi_scale;
+    f[8] = (pi + xj.w) * (0.5f * vj.w) * nm.pi_scale;
 }
 
 template <int SUB, bool EMIT>
 __global__ __launch_bounds__(256) void k_pairs(const SceneDev* __restrict__ scenes, int nscenes,
                                                const uint32_t* __restrict__ start, const float4* __restrict__ P,
                                                const float4* __restrict__ V, const float* __restrict__ rho,
-                                               const float2* __restrict__ D2, uint32_t first, uint32_t stride,
+                                               uint32_t first, uint32_t stride,
                                                uint32_t T, MlpNorm nm, uint32_t* __restrict__ cnt,
                                                const uint64_t* __restrict__ off, uint32_t* __restrict__ pj,
                                                uint32_t* __restrict__ pt, float* __restrict__ X)
@@ -98,7 +99,7 @@ __global__ __launch_bounds__(256) void k_pairs(const SceneDev* __restrict__ scen
     if constexpr (EMIT) {
         vi = V[s];
         rhoi = rho[s];
-        pi = D2[s].x;
+        pi = xi.w;
     }
     int cx, cy, cz;
     float fx, fy, fz;
@@ -111,7 +112,7 @@ __global__ __launch_bounds__(256) void k_pairs(const SceneDev* __restrict__ scen
                 if (r2 < h2 && j != s) {
                     if constexpr (EMIT) {
                         float f[kNin];
-                        pair_features(sc, nm, xi, vi, rhoi, pi, xj, V[j], rho[j], D2[j], f);
+                        pair_features(sc, nm, xi, vi, rhoi, pi, xj, V[j], rho[j], f);
 #pragma unroll
                         for (int k = 0; k < kNin; ++k) X[row * kNin + k] = f[k];
                         pj[row] = j;
@@ -378,7 +379,7 @@ __global__ __launch_bounds__(256) void k_train_targets(const SceneDev* __restric
 __global__ __launch_bounds__(kTile) void k_coarse_integrate(const Tile* __restrict__ tiles,
                                                             const SceneDev* __restrict__ scenes,
                                                             const float4* __restrict__ PA, const float4* __restrict__ VA,
-                                                            const float* __restrict__ rho, const float2* __restrict__ D2,
+                                                            const float* __restrict__ rho,
                                                             const float* __restrict__ pred,
                                                             const uint32_t* __restrict__ perm,
                                                             sph_particle_t* __restrict__ aos, float4* __restrict__ PB,
@@ -424,7 +425,7 @@ __global__ __launch_bounds__(kTile) void k_coarse_integrate(const Tile* __restri
     r[9] = rh * a[1];
     r[10] = rh * a[2];
     r[12] = rh;
-    r[13] = D2[s].x;
+    r[13] = x.w;
 }
 
 // Summed messages of every slot, FP32, one particle per lane and the whole MLP
@@ -435,7 +436,7 @@ __global__ __launch_bounds__(256) void k_coarse_mlp_fp32(const Tile* __restrict_
                                                          const SceneDev* __restrict__ scenes,
                                                          const uint32_t* __restrict__ start,
                                                          const float4* __restrict__ P, const float4* __restrict__ V,
-                                                         const float* __restrict__ rho, const float2* __restrict__ D2,
+                                                         const float* __restrict__ rho,
                                                          const float* __restrict__ params, MlpNorm nm,
                                                          float* __restrict__ pred)
 {
@@ -447,7 +448,7 @@ __global__ __launch_bounds__(256) void k_coarse_mlp_fp32(const Tile* __restrict_
     const uint32_t s = t.begin + threadIdx.x;
     if (s >= t.end) return;
     const float4 xi = P[s], vi = V[s];
-    const float rhoi = rho[s], pi = D2[s].x;
+    const float rhoi = rho[s], pi = xi.w;
     const float h2 = sc.h2;
     float acc[kNout] = {0.f, 0.f, 0.f};
     int cx, cy, cz;
@@ -460,7 +461,7 @@ __global__ __launch_bounds__(256) void k_coarse_mlp_fp32(const Tile* __restrict_
                 const float r2 = fmaf(ddx, ddx, fmaf(ddy, ddy, ddz * ddz));
                 if (!(r2 < h2) || j == s) continue;
                 float f[kNin];
-                pair_features(sc, nm, xi, vi, rhoi, pi, xj, V[j], rho[j], D2[j], f);
+                pair_features(sc, nm, xi, vi, rhoi, pi, xj, V[j], rho[j], f);
                 float h1[kNh];
 #pragma unroll
                 for (int u = 0; u < kNh; ++u) {
@@ -576,7 +577,7 @@ template <int SUB>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k_coarse_mlp_mfma(const SceneDev* __restrict__ scenes, int nscenes,
                                                          const uint32_t* __restrict__ start,
                                                          const float4* __restrict__ P, const float4* __restrict__ V,
-                                                         const float* __restrict__ rho, const float2* __restrict__ D2,
+                                                         const float* __restrict__ rho,
                                                          const __bf16* __restrict__ pa, const float* __restrict__ pc,
                                                          MlpNorm nm, uint32_t n, float* __restrict__ pred,
                                                          unsigned long long* __restrict__ pair_count)
@@ -633,7 +634,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k
         const uint32_t s = (uint32_t)__builtin_amdgcn_readfirstlane((int)si);
         const SceneDev& sc = scenes[scene_of(scenes, nscenes, s)];
         const float4 xi = P[s], vi = V[s];
-        const float rhoi = rho[s], pi = D2[s].x;
+        const float rhoi = rho[s], pi = xi.w;
         const float h2 = sc.h2, inv_h = 1.0f / sc.h;
         float acc0 = 0.f, acc1 = 0.f, acc2 = 0.f;
         int nh = 0;
@@ -664,8 +665,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k
                     bx[6] = (__bf16)(rhoi * nm.rho_scale);
                     bx[7] = (__bf16)(rj * nm.rho_scale);
                 } else if (lane < 32) {
-                    const float2 dj = D2[j];
-                    bx[0] = (__bf16)((pi + dj.x) * (0.5f * dj.y) * nm.pi_scale);
+                    const float pj = P[j].w, inv_rhoj = V[j].w;
+                    bx[0] = (__bf16)((pi + pj) * (0.5f * inv_rhoj) * nm.pi_scale);
                 }
                 f32x4 h1[4];
 #pragma unroll
@@ -822,10 +823,10 @@ hipError_t launch_pairs(hipStream_t st, const DeviceState& d, int nscenes, const
 #define NPRSPH_PAIRS(S)                                                                                          \
     if (emit)                                                                                                    \
         hipLaunchKernelGGL((k_pairs<S, true>), grid, block, 0, st, d.scenes, nscenes, d.start, d.PA, d.VA, d.rho, \
-                           d.D2, first, stride, T, w.norm, w.cnt, w.off, w.pj, w.pt, w.X);                       \
+                           first, stride, T, w.norm, w.cnt, w.off, w.pj, w.pt, w.X);                       \
     else                                                                                                         \
         hipLaunchKernelGGL((k_pairs<S, false>), grid, block, 0, st, d.scenes, nscenes, d.start, d.PA, d.VA, d.rho, \
-                           d.D2, first, stride, T, w.norm, w.cnt, w.off, w.pj, w.pt, w.X);
+                           first, stride, T, w.norm, w.cnt, w.off, w.pj, w.pt, w.X);
     if (sub == 1) {
         NPRSPH_PAIRS(1)
     } else if (sub == 2) {
@@ -888,7 +889,7 @@ hipError_t launch_train_targets(hipStream_t st, const DeviceState& d, int nscene
 hipError_t launch_coarse_integrate(hipStream_t st, const DeviceState& d, const MlpWork& w, bool use_force)
 {
     hipLaunchKernelGGL(k_coarse_integrate, dim3(d.ntiles), dim3(kTile), 0, st, d.tiles, d.scenes, d.PA, d.VA, d.rho,
-                       d.D2, w.pred, d.perm, d.aos, d.PB, d.VB, w.norm.acc_scale, use_force ? 1 : 0);
+                       w.pred, d.perm, d.aos, d.PB, d.VB, w.norm.acc_scale, use_force ? 1 : 0);
     return hipGetLastError();
 }
 
@@ -906,7 +907,7 @@ hipError_t launch_coarse_mlp(hipStream_t st, const DeviceState& d, int nscenes, 
         if (e != hipSuccess) return e;
         const dim3 grid(std::min<uint32_t>((d.n + 3u) / 4u, 256u * 64u)), block(256);
 #define NPRSPH_MFMA(S)                                                                                             \
-    hipLaunchKernelGGL(k_coarse_mlp_mfma<S>, grid, block, 0, st, d.scenes, nscenes, d.start, d.PA, d.VA, d.rho, d.D2, \
+    hipLaunchKernelGGL(k_coarse_mlp_mfma<S>, grid, block, 0, st, d.scenes, nscenes, d.start, d.PA, d.VA, d.rho, \
                        w.pack_a, w.pack_c, w.norm, d.n, w.pred, w.pairs)
         if (sub == 1)
             NPRSPH_MFMA(1);
@@ -919,13 +920,13 @@ hipError_t launch_coarse_mlp(hipStream_t st, const DeviceState& d, int nscenes, 
     }
     if (sub == 1)
         hipLaunchKernelGGL(k_coarse_mlp_fp32<1>, dim3(d.ntiles), dim3(256), 0, st, d.tiles, d.scenes, d.start, d.PA,
-                           d.VA, d.rho, d.D2, w.params, w.norm, w.pred);
+                           d.VA, d.rho, w.params, w.norm, w.pred);
     else if (sub == 2)
         hipLaunchKernelGGL(k_coarse_mlp_fp32<2>, dim3(d.ntiles), dim3(256), 0, st, d.tiles, d.scenes, d.start, d.PA,
-                           d.VA, d.rho, d.D2, w.params, w.norm, w.pred);
+                           d.VA, d.rho, w.params, w.norm, w.pred);
     else
         hipLaunchKernelGGL(k_coarse_mlp_fp32<3>, dim3(d.ntiles), dim3(256), 0, st, d.tiles, d.scenes, d.start, d.PA,
-                           d.VA, d.rho, d.D2, w.params, w.norm, w.pred);
+                           d.VA, d.rho, w.params, w.norm, w.pred);
     return hipGetLastError();
 }
 

@@ -253,8 +253,11 @@ def test_event_timed_steps_equal_graph_steps(pkg, orc, gpu_lib):
             sim.step(19)
         assert timed.download().tobytes() == graphed.download().tobytes()
         times = timed.kernel_times()
-        for k in ("hash_count", "scan_reduce", "scan_down", "scatter", "rank_reorder", "density"):
+        for k in ("scan_reduce", "scan_down", "scatter", "rank_reorder", "density"):
             assert times[k][1] == 41, (k, times[k])  # 40 steps and the frame's rho_pres
+        # The fused step counts the next grid's cells itself: only the builds
+        # after an upload or a three-pass integrate run the separate count.
+        assert times["hash_count"][1] == 2, times["hash_count"]
         assert "scan_top" not in times  # a grid this small folds the block sums into scan_down
         assert times["force_integrate"][1] == 40
         assert times["force"][1] == 1 and times["integrate"][1] == 1
@@ -412,3 +415,45 @@ def test_staged_rows_every_tuning(pkg, orc, gpu_lib, sub):
     check_density(orc, st, s1, ob, grid=False)
     check_force(orc, s1, s2, ob, grid=False)
     check_integrate(orc, s2, s3, ob)
+
+
+def test_counts_left_by_the_fused_step(pkg, orc, gpu_lib):
+    """The fused step counts its new state into the next grid's cells.  Those
+    counts serve the next build whichever call makes it (another step, the
+    three-pass surface, a coarse step), and are discarded when the state or
+    the grid is replaced (upload, regrid).  Every path equals a fresh context
+    fed the same AoS state."""
+    sc = pkg.scenes.dam10k()
+    p0 = _lattice(orc, sc)
+    b = pkg.sph.make_boundary(sc.lower, sc.upper)
+
+    def fresh_passes(state):
+        with pkg.Simulation(sc.count, boundaries=b) as f:
+            f.upload(state)
+            return _passes(f)
+
+    with pkg.Simulation(sc.count, boundaries=b) as sim:
+        sim.upload(p0)
+        sim.step(5)
+        st = sim.download()
+        got = _passes(sim)  # the build reuses the fused step's counts
+        for x, y in zip(got, fresh_passes(st)):
+            assert x.tobytes() == y.tobytes()
+        sim.step(3)
+        sim.upload(p0)  # counted state dropped: stale counts must not leak into the next build
+        sim.step(4)
+        with pkg.Simulation(sc.count, boundaries=b) as ref:
+            ref.upload(p0)
+            ref.step(4)
+            assert sim.download().tobytes() == ref.download().tobytes()
+        sim.step(2)
+        sim.set_tuning(3, 1, 128, 0)  # regrid after a counting step
+        sim.step(2)
+        st = sim.download()
+        sim.set_tuning(2, 1, 128, 0)
+        sim.upload(st)
+        sim.step(3)
+        with pkg.Simulation(sc.count, boundaries=b) as ref:
+            ref.upload(st)
+            ref.step(3)
+            assert sim.download().tobytes() == ref.download().tobytes()

@@ -506,7 +506,7 @@ void free_all(sph_ctx* c)
                     c->d.slot, c->d.torig, c->d.tsrc, c->d.tkey, c->d.count, c->d.start, c->d.bsum, c->d.partials,
                     c->d_scenes, c->d_tiles, c->d_tiles128, c->d_backup, c->d_judge_s, c->d_judge_p, c->d_pred_aos,
                     w.params, w.adam_m, w.adam_v, w.grad, w.cnt, w.off, w.tgt, w.wt, w.pred, w.dpred, w.lossp,
-                    w.pj, w.pt, w.X, w.Z1, w.Z2, w.dZ1, w.dZ2, w.M, w.partial, w.pack_a, w.pack_c, w.pairs};
+                    w.pj, w.pt, w.X, w.Z1, w.Z2, w.dZ1, w.dZ2, w.M, w.partial, w.pack_a, w.pack_c, w.pairs, w.gelu_tab};
     for (void* p : bufs)
         if (p) (void)hipFree(p);
     if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
@@ -538,6 +538,7 @@ int ensure_mlp_base(sph_ctx* c)
     if (dalloc(&w.params, kMlpParams) != hipSuccess || dalloc(&w.adam_m, kMlpParams) != hipSuccess ||
         dalloc(&w.adam_v, kMlpParams) != hipSuccess || dalloc(&w.grad, kMlpParams) != hipSuccess ||
         dalloc(&w.pack_a, kMlpPackA) != hipSuccess || dalloc(&w.pack_c, kMlpPackC) != hipSuccess ||
+        dalloc(&w.gelu_tab, kMlpGeluTab) != hipSuccess ||
         dalloc(&w.pairs, 1) != hipSuccess) {
         (void)hipGetLastError();
         return fail(c, SPH_ERR_NOMEM, "cannot allocate network parameters");

@@ -99,12 +99,14 @@ struct MlpWork {
     float* partial = nullptr;  // kMlpParams per gradient block
     __bf16* pack_a = nullptr;  // MFMA operand images (k_mlp_pack)
     float* pack_c = nullptr;
+    uint16_t* gelu_tab = nullptr;  // BF16 GELU table of the MFMA kernel (k_gelu_table)
     unsigned long long* pairs = nullptr;  // pairs evaluated by the last MFMA message pass
     bool pack_stale = true;    // params changed since the last pack
 };
 
 constexpr int kMlpPackA = 4 * 64 * 8 + 4 * 2 * 64 * 8 + 2 * 64 * 8;  // bf16 entries
 constexpr int kMlpPackC = 4 * 64 * 4 * 2 + 64 * 4;                  // float entries
+constexpr int kMlpGeluTab = 4096;                                    // GELU table entries
 
 // Each launcher enqueues one kernel on `s` and returns hipGetLastError().
 hipError_t launch_set_scene(hipStream_t s, SceneDev* dst, const SceneDev& value);

@@ -504,26 +504,45 @@ __global__ __launch_bounds__(256) void k_coarse_mlp_fp32(const Tile* __restrict_
 // weights (k_mlp_pack).
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
-typedef _Float16 f16x2 __attribute__((ext_vector_type(2)));
 
-// gelu_poly of two values in packed FP16 arithmetic, rounded to BF16 for the
-// next MFMA.  Written with c = clamp(x, -3, 3):
-//   0.5c + q(c^2) + max(x - c, 0),
-// which equals gelu_poly for every finite x (q(9) - 1.5 is the same offset
-// beyond the knee in both) and stays finite for x = -inf.  FP16 carries 11
-// significant bits against BF16's 8, so the result is the BF16 rounding of
-// gelu_poly to within one BF16 step.
 typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
-__device__ __forceinline__ bf16x2 gelu2_bf16(float a, float b)
+
+// GELU of the MFMA kernel by table lookup.  An activation is converted to FP16
+// (round toward zero) and the top kGeluBits bits of its FP16 pattern (sign, 5
+// exponent bits, 6 mantissa bits) index the table; entry i holds gelu_poly at
+// the centre of that bucket, rounded to BF16 (the next MFMA's operand type).
+// Over the whole FP16 range this stays within about one BF16 step of
+// gelu_poly, the training-time form, at one conversion, two shifts and two
+// LDS reads per pair of activations.  The 4,096-entry table (8 KB of LDS)
+// keeps 6 workgroups of 4 waves per CU; a 13-bit table would halve that.
+constexpr int kGeluBits = 12;
+constexpr int kGeluTab = 1 << kGeluBits;
+constexpr int kGeluShift = 16 - kGeluBits;
+static_assert(kGeluTab <= kMlpGeluTab, "table buffer");
+
+__global__ __launch_bounds__(256) void k_gelu_table(uint16_t* __restrict__ tab)
 {
-    const f16x2 x = {(_Float16)a, (_Float16)b};
-    const f16x2 knee = (_Float16)kGeluKnee;
-    const f16x2 c = __builtin_elementwise_min(__builtin_elementwise_max(x, -knee), knee);
-    const f16x2 sq = c * c;
-    const f16x2 q = ((((f16x2)(_Float16)kGeluC3 * sq + (_Float16)kGeluC2) * sq + (_Float16)kGeluC1) * sq +
-                     (_Float16)kGeluC0) * sq;
-    const f16x2 g = (f16x2)(_Float16)0.5f * c + q + __builtin_elementwise_max(x - c, (f16x2)(_Float16)0.f);
-    return bf16x2{(__bf16)(float)g.x, (__bf16)(float)g.y};
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= (uint32_t)kGeluTab) return;
+    const uint32_t bits = (i << kGeluShift) | (1u << (kGeluShift - 1));  // bucket centre
+    const uint32_t expo = (bits >> 10) & 31u, sign = bits >> 15;
+    float g;
+    if (expo == 31u) {
+        // Infinities (and the NaNs sharing their bucket): GELU(+inf) = +inf,
+        // GELU(-inf) = 0; the other buckets of this exponent are NaN.
+        const bool inf_bucket = ((bits & 0x3ffu) >> kGeluShift) == 0u;
+        g = inf_bucket ? (sign ? 0.f : __builtin_inff()) : __builtin_nanf("");
+    } else {
+        g = gelu_poly((float)__builtin_bit_cast(_Float16, (uint16_t)bits));
+    }
+    tab[i] = __builtin_bit_cast(uint16_t, (__bf16)g);
+}
+
+__device__ __forceinline__ bf16x2 gelu2_tab(const uint16_t* tab, float a, float b)
+{
+    const uint32_t h = __builtin_bit_cast(uint32_t, __builtin_amdgcn_cvt_pkrtz(a, b));
+    const uint32_t lo = tab[(h & 0xffffu) >> kGeluShift], hi = tab[h >> (16 + kGeluShift)];
+    return __builtin_bit_cast(bf16x2, lo | (hi << 16));
 }
 
 __host__ __device__ __forceinline__ int kslot_unit(int kslot)
@@ -570,9 +589,10 @@ constexpr int kHitCap = 192;  // hit list entries per wave
 // Summed messages of every slot on MFMA: one wave per target particle.  The
 // wave scans the target's stencil rows 64 candidates at a time, compacts the
 // hits into an LDS list, and runs the network on 16-pair tiles.  With the
-// network's operands in LDS a wave fits 64 registers, so 8 waves share each
-// SIMD (18 KB of LDS per 4-wave workgroup): the GELU arithmetic of one wave
-// issues while others wait on their MFMA results and loads.
+// network's operands in LDS a wave fits 64 registers; the operands, hit lists
+// and GELU table take 26 KB of LDS per 4-wave workgroup, so 6 waves share each
+// SIMD: the GELU lookups of one wave issue while others wait on their MFMA
+// results and loads.
 template <int SUB>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k_coarse_mlp_mfma(const SceneDev* __restrict__ scenes, int nscenes,
                                                          const uint32_t* __restrict__ start,
@@ -580,9 +600,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k
                                                          const float* __restrict__ rho,
                                                          const __bf16* __restrict__ pa, const float* __restrict__ pc,
                                                          MlpNorm nm, uint32_t n, float* __restrict__ pred,
-                                                         unsigned long long* __restrict__ pair_count)
+                                                         unsigned long long* __restrict__ pair_count,
+                                                         const uint16_t* __restrict__ gtab)
 {
     __shared__ uint32_t hits_all[4][kHitCap];
+    // The GELU table (k_gelu_table), staged once per workgroup.
+    __shared__ uint4 s_gelu4[kGeluTab / 8];
+    const uint16_t* s_gelu = reinterpret_cast<const uint16_t*>(s_gelu4);
+    for (int i = (int)threadIdx.x; i < kGeluTab / 8; i += 256) s_gelu4[i] = reinterpret_cast<const uint4*>(gtab)[i];
     // The network's MFMA operands (weights and biases), staged once per
     // workgroup as per-lane images and read back at each MFMA, so that a
     // wave's registers hold only its working set (more waves per SIMD).
@@ -679,7 +704,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k
                     for (int mm = 0; mm < 2; ++mm)
 #pragma unroll
                         for (int e = 0; e < 4; e += 2) {
-                            const bf16x2 g = gelu2_bf16(h1[2 * ks + mm][e], h1[2 * ks + mm][e + 1]);
+                            const bf16x2 g = gelu2_tab(s_gelu, h1[2 * ks + mm][e], h1[2 * ks + mm][e + 1]);
                             b2[ks][4 * mm + e] = g.x;
                             b2[ks][4 * mm + e + 1] = g.y;
                         }
@@ -696,7 +721,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k
                     for (int mm = 0; mm < 2; ++mm)
 #pragma unroll
                         for (int e = 0; e < 4; e += 2) {
-                            const bf16x2 g = gelu2_bf16(h2[2 * ks + mm][e], h2[2 * ks + mm][e + 1]);
+                            const bf16x2 g = gelu2_tab(s_gelu, h2[2 * ks + mm][e], h2[2 * ks + mm][e + 1]);
                             b3[ks][4 * mm + e] = g.x;
                             b3[ks][4 * mm + e + 1] = g.y;
                         }
@@ -896,6 +921,7 @@ hipError_t launch_coarse_integrate(hipStream_t st, const DeviceState& d, const M
 hipError_t launch_mlp_pack(hipStream_t st, const MlpWork& w)
 {
     hipLaunchKernelGGL(k_mlp_pack, dim3(1), dim3(256), 0, st, w.params, w.pack_a, w.pack_c);
+    hipLaunchKernelGGL(k_gelu_table, dim3((kGeluTab + 255) / 256), dim3(256), 0, st, w.gelu_tab);
     return hipGetLastError();
 }
 
@@ -908,7 +934,7 @@ hipError_t launch_coarse_mlp(hipStream_t st, const DeviceState& d, int nscenes, 
         const dim3 grid(std::min<uint32_t>((d.n + 3u) / 4u, 256u * 64u)), block(256);
 #define NPRSPH_MFMA(S)                                                                                             \
     hipLaunchKernelGGL(k_coarse_mlp_mfma<S>, grid, block, 0, st, d.scenes, nscenes, d.start, d.PA, d.VA, d.rho, \
-                       w.pack_a, w.pack_c, w.norm, d.n, w.pred, w.pairs)
+                       w.pack_a, w.pack_c, w.norm, d.n, w.pred, w.pairs, w.gelu_tab)
         if (sub == 1)
             NPRSPH_MFMA(1);
         else if (sub == 2)

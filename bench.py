@@ -185,6 +185,10 @@ def learned_form(pkg, scenes, state, first_step, fine_steps, plain_ms_per_step, 
     s0 = state
     sim.upload(s0)
     sim.coarse_step(1, **judge)  # kernel warm-up
+    # Neighbour pairs (r < h, j != i) of the starting state: the message pass
+    # of one unjudged coarse step counts them.
+    sim.upload(s0)
+    pairs0 = sim.coarse_step(1, use_network=True, judge_every=0, drift_limit=0.0, cooldown=0).pairs
     sim.upload(s0)
     sim.prepare()
     sim.synchronize()
@@ -265,6 +269,7 @@ def learned_form(pkg, scenes, state, first_step, fine_steps, plain_ms_per_step, 
         "mfma_roofline": mfma,
         "judge": st.as_dict(),
         "first_step": first_step,
+        "pairs_at_first_step": int(pairs0),
         "drift_definition": f"judge every {JUDGE_EVERY} coarse steps: |mean rho(1 coarse step) - mean rho(4 plain "
                             f"steps)| / mean rho(4 plain steps) from the same state; above {DRIFT_LIMIT} the plain "
                             f"result is kept and the next {JUDGE_EVERY - 1} coarse steps run plain before the next "
@@ -296,6 +301,58 @@ def profile_record(workload, kernel, tuning, settle):
     if rec.get("kernel") != kernel or rec.get("tuning") != tuning or rec.get("settle") != settle:
         return None
     return rec
+
+
+# Useful FP32 flops per particle of the gathers (the flop model of the
+# three-pass path): C candidates scanned, K neighbours within h.
+#   density: 9 per candidate (distance and kernel term) + 6 per neighbour
+#   force:   9 per candidate + 35 per neighbour (pressure and viscosity terms)
+FP32_PEAK_TFLOPS = 157.3
+
+
+def gather_counts(sim, state, scene=0):
+    """Mean stencil candidates per particle of `state` (scene `scene`): the
+    slot ranges the gathers scan, from the neighbour grid the library builds
+    (cell starts) and the pruning rule of its row walk, restated in numpy.
+    Diagnostic only: it feeds the useful-flop figures of the line."""
+    sim.upload(state)
+    sim.rho_pres()
+    start, perm = sim.download_grid()
+    gi = sim.grid_info(scene)
+    sub = gi.cells_per_h
+    prune = sim.tuning().prune
+    first, count = gi.first, gi.count
+    pos = state["pos"][first:first + count, :3].astype(np.float32)
+    lo = np.array(gi.lo[:3], dtype=np.float32)
+    dims = np.array(gi.dims[:3])
+    f = (pos - lo) * np.float32(gi.inv_cell)
+    ok = ~np.isnan(f).any(axis=1)
+    f = np.where(ok[:, None], f, 0.0).astype(np.float32)
+    c = np.clip(np.floor(f), 0, dims - 1).astype(np.int64)
+    c = np.where(f < 0, 0, np.where(f >= dims.astype(np.float32), dims - 1, c))
+    dx, dy, dz = (int(v) for v in dims)
+    xa = np.maximum(c[:, 0] - sub, 0)
+    xb = np.minimum(c[:, 0] + sub, dx - 1)
+    total = np.zeros(count, dtype=np.int64)
+    base = gi.cell_base
+    for oz in range(-sub, sub + 1):
+        z = c[:, 2] + oz
+        gz = np.where(z > c[:, 2], z - f[:, 2], np.where(z < c[:, 2], f[:, 2] - (z + 1), 0.0))
+        for oy in range(-sub, sub + 1):
+            y = c[:, 1] + oy
+            inb = ok & (z >= 0) & (z < dz) & (y >= 0) & (y < dy)
+            x0, x1 = xa, xb
+            if prune:
+                gy = np.where(y > c[:, 1], y - f[:, 1], np.where(y < c[:, 1], f[:, 1] - (y + 1), 0.0))
+                rem = sub * sub - (gy * gy + gz * gz)
+                inb &= rem > 0
+                w = np.sqrt(np.maximum(rem, 0.0))
+                x0 = np.maximum(np.minimum(np.floor(f[:, 0] - w), dx - 1), xa).astype(np.int64)
+                x1 = np.minimum(np.maximum(np.floor(f[:, 0] + w), 0), xb).astype(np.int64)
+            row = base + (np.clip(z, 0, dz - 1) * dy + np.clip(y, 0, dy - 1)) * dx
+            nrow = start[row + x1 + 1].astype(np.int64) - start[row + x0].astype(np.int64)
+            total += np.where(inb, nrow, 0)
+    return float(total.mean())
 
 
 def three_passes(sim, n, frames: int = 5):
@@ -477,11 +534,15 @@ def main(argv=None):
                     # committed counter runs of this workload, state and tuning
                     "profile": None if prof is None else prof.get("source"),
                     "profile_launch_ms": None if prof is None else prof.get("rocprof_avg_ms_timed_launches"),
+                    # fraction of SIMD quad-cycles issuing a VALU instruction, of
+                    # the dual-issue ceiling, and active lanes per VALU instruction
                     "valu_busy": None if prof is None else prof.get("valu_busy"),
+                    "valu_issue_slots": None if prof is None else prof.get("valu_issue_slots"),
                     "valu_lane_utilization": None if prof is None else prof.get("valu_lane_utilization"),
-                    "useful_fp32_tflops": None if prof is None else prof.get("useful_fp32_tflops"),
-                    "binding_roof": "VALU issue: the neighbour gathers are instruction-bound (candidate scans and "
-                                    "pair terms on the vector ALUs), with HBM mostly idle"}
+                    "density": None if prof is None else prof.get("density"),
+                    "binding_roof": "VALU issue and latency: the neighbour gathers are instruction-bound (candidate "
+                                    "scans and pair terms on the vector ALUs, 39% of lanes idle in divergent "
+                                    "loops), with HBM mostly idle"}
     # Host readback of the whole AoS (sph_download into pageable numpy memory):
     # what a frame would cost a caller that takes every step's particles to the
     # host instead of binding the device buffer.  Untimed region; not `value`.
@@ -502,6 +563,7 @@ def main(argv=None):
              "peak": HBM_PEAK_GBS, "unit": "GB/s"}
     if build["achieved"] is not None:
         build["frac"] = build["achieved"] / HBM_PEAK_GBS
+    cand = gather_counts(sim, region_start) if len(scenes) == 1 else None
     sim.upload(region_start)
     passes = three_passes(sim, n)
     sim.close()
@@ -511,6 +573,17 @@ def main(argv=None):
     if not args.no_learned and len(scenes) == 1:
         learned = learned_form(pkg, scenes, region_start, first_step, args.steps, ms_per_step)
     del region_start
+
+    # Useful FP32 rate of the gathers under the flop model, with the measured
+    # candidates (C) and neighbours (K) per particle of the timed state.
+    if roofline is not None and cand is not None and learned is not None:
+        nb = learned["pairs_at_first_step"] / n
+        work = {"candidates_per_particle": cand, "neighbours_per_particle": nb}
+        for k, fl in (("density", 9 * cand + 6 * nb), ("force_integrate", 9 * cand + 35 * nb)):
+            if k in kern:
+                tf = fl * n / (kern[k]["ms_avg"] * 1e-3) / 1e12
+                work[k] = {"flops_per_particle": fl, "tflops": tf, "frac_of_fp32_peak": tf / FP32_PEAK_TFLOPS}
+        roofline["useful_fp32"] = work
 
     cpu = None
     if args.cpu_budget > 0:

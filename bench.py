@@ -70,13 +70,15 @@ LEARNED_WINDOW = 10
 # Learned message pass: flops of the pair MLP 9 -> 64 -> 64 -> 3 per neighbour
 # pair (multiply-adds counted as 2), against MI355X's dense BF16 MFMA peak.
 MLP_FLOPS_PER_PAIR = 2 * (9 * 64 + 64 * 64 + 64 * 3)
-# Neighbour build (no reference counterpart), compulsory bytes per particle:
-#   hash_count    pos.xyz 12 read; key, in-cell rank 8 written             = 20
+# Neighbour build (no reference counterpart), compulsory bytes per particle of
+# the kernels each step launches (the cell key and count of every particle are
+# taken by the previous step's fused force kernel; k_hash_count runs only
+# after an upload):
 #   scatter       key, rank, cell start, index 16 read; 3 index arrays 12 written = 28
 #   rank_reorder  3 index arrays 12 + pos, vel 24 read; index 4 + pos, vel 24 written = 64
 # (the per-cell count/scan arrays are per cell, not per particle, and left out)
-BUILD_KERNELS = ("hash_count", "scan_reduce", "scan_top", "scan_down", "scatter", "rank_reorder")
-BUILD_BYTES = 20 + 28 + 64
+BUILD_KERNELS = ("scan_reduce", "scan_top", "scan_down", "scatter", "rank_reorder")
+BUILD_BYTES = 28 + 64
 BF16_DENSE_TFLOPS = 2500.0
 
 
@@ -234,7 +236,7 @@ def learned_form(pkg, scenes, state, first_step, fine_steps, plain_ms_per_step, 
               "plain_ms_per_step": win_plain_ms, "speedup_vs_plain_network_only": 4.0 * win_plain_ms / win_coarse_ms}
     progress(f"learned: {ncoarse} coarse steps in {el:.2f} s; accuracy timelines")
     # accuracy: mean density every `every` fine steps of simulated time
-    every = 20 if n <= 2_000_000 else 100
+    every = min(20 if n <= 2_000_000 else 100, 4 * ncoarse)
     sim.upload(s0)
     plain, ke_p, com_p = density_timeline(sim, fine_steps, every)
     progress("learned: plain timeline done")

@@ -588,13 +588,15 @@ constexpr int kHitCap = 192;  // hit list entries per wave
 
 // Summed messages of every slot on MFMA: one wave per target particle.  The
 // wave scans the target's stencil rows 64 candidates at a time, compacts the
-// hits into an LDS list, and runs the network on 16-pair tiles.  With the
-// network's operands in LDS a wave fits 64 registers; the operands, hit lists
-// and GELU table take 26 KB of LDS per 4-wave workgroup, so 6 waves share each
-// SIMD: the GELU lookups of one wave issue while others wait on their MFMA
-// results and loads.
+// hits into an LDS list, and runs the network on 16-pair tiles.  The network's
+// operands, the hit lists and the GELU table take 26 KB of LDS per 4-wave
+// workgroup, so 6 waves share each SIMD: the GELU lookups of one wave issue
+// while others wait on their MFMA results and loads.  The register budget of
+// 6 waves (85 per lane) lets the MFMA results land in VGPRs, where the GELU
+// reads them; at a budget of 64 they went through AGPRs and cost 35 extra
+// moves per 16-pair tile.
 template <int SUB>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k_coarse_mlp_mfma(const SceneDev* __restrict__ scenes, int nscenes,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k_coarse_mlp_mfma(const SceneDev* __restrict__ scenes, int nscenes,
                                                          const uint32_t* __restrict__ start,
                                                          const float4* __restrict__ P, const float4* __restrict__ V,
                                                          const float* __restrict__ rho,

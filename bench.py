@@ -287,16 +287,21 @@ def learned_form(pkg, scenes, state, first_step, fine_steps, plain_ms_per_step, 
     }
 
 
+def profile_tag(tuning):
+    """File-name tag of a tuning: t<cells_per_h>-<prune>-<block>-<blocks_per_xcd>."""
+    return "t{cells_per_h}-{prune}-{block}-{blocks_per_xcd}".format(**tuning)
+
+
 def profile_record(workload, kernel, tuning, settle):
     """The committed rocprofv3 record of `kernel` for this workload, settled
-    state and tuning (profiles/traffic_<workload>.json), or None.  It holds the
+    state and tuning (profiles/traffic_<workload>_<tuning tag>.json), or None.  It holds the
     kernel's HBM bytes per launch (FETCH_SIZE and WRITE_SIZE, each calibrated
     by the same counter over a plain copy of known size) and counter-derived
     utilisations: VALU busy (SQ_ACTIVE_INST_VALU over CU count x
     GRBM_GUI_ACTIVE, rocprofv3's VALUBusy), the active-lane fraction of VALU
     instructions (SQ_THREAD_CYCLES_VALU over 64 x SQ_ACTIVE_INST_VALU,
     rocprofv3's VALUUtilization) and the useful FP32 rate of the flop model."""
-    path = os.path.join(ROOT, "profiles", f"traffic_{workload}.json")
+    path = os.path.join(ROOT, "profiles", f"traffic_{workload}_{profile_tag(tuning)}.json")
     if not os.path.exists(path):
         return None
     rec = json.load(open(path))
@@ -304,6 +309,10 @@ def profile_record(workload, kernel, tuning, settle):
         return None
     return rec
 
+
+# The library keys cells in bands of this many z-planes:
+# key = ((z / band * dy + y) * band + z % band) * dx + x.
+CELL_BAND = 8
 
 # Useful FP32 flops per particle of the gathers (the flop model of the
 # three-pass path): C candidates scanned, K neighbours within h.
@@ -351,7 +360,8 @@ def gather_counts(sim, state, scene=0):
                 w = np.sqrt(np.maximum(rem, 0.0))
                 x0 = np.maximum(np.minimum(np.floor(f[:, 0] - w), dx - 1), xa).astype(np.int64)
                 x1 = np.minimum(np.maximum(np.floor(f[:, 0] + w), 0), xb).astype(np.int64)
-            row = base + (np.clip(z, 0, dz - 1) * dy + np.clip(y, 0, dy - 1)) * dx
+            zc, yc = np.clip(z, 0, dz - 1), np.clip(y, 0, dy - 1)
+            row = base + (((zc // CELL_BAND) * dy + yc) * CELL_BAND + zc % CELL_BAND) * dx
             nrow = start[row + x1 + 1].astype(np.int64) - start[row + x0].astype(np.int64)
             total += np.where(inb, nrow, 0)
     return float(total.mean())

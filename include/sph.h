@@ -101,7 +101,9 @@ typedef struct sph_grid_info {
     float inv_cell;       /* 1 / cell edge (float32, used by the cell hash) */
     double cell;          /* cell edge (>= smoothing length / cells_per_h) */
     int32_t dims[3];      /* cells per axis */
-    uint32_t ncells;      /* dims[0]*dims[1]*dims[2]; cell `ncells` holds NaN positions */
+    uint32_t ncells;      /* dims[0]*dims[1]*8*ceil(dims[2]/8): cells are keyed band by band,
+                             key = ((z/8*dims[1] + y)*8 + z%8)*dims[0] + x; cell `ncells`
+                             holds NaN positions */
     uint32_t cell_base;   /* global index of this scene's first cell */
     uint32_t first;       /* first particle of this scene in the AoS buffer */
     uint32_t count;       /* particles in this scene */

@@ -41,7 +41,8 @@ struct GridGeom {
 
 // Cell edge h*(1+2^-10)/sub: with float32 rounding of the hash, |xi-xj| < h
 // still lands within `sub` cells on every axis.  Grown by 2^(1/4) until each
-// axis has <= 2048 cells and the grid has <= max(2^19, 16n) cells.
+// axis has <= 2048 cells and the grid, z rounded up to whole bands of
+// kCellBand planes, has <= max(2^19, 16n) cells.
 bool grid_geometry(float h, const sph_boundary_t& b, uint64_t n, int sub, GridGeom& g)
 {
     if (!(h > 0.0f) || !std::isfinite(h) || sub < 1 || sub > 3) return false;
@@ -61,7 +62,7 @@ bool grid_geometry(float h, const sph_boundary_t& b, uint64_t n, int sub, GridGe
                 d = 2048.0;
             }
             g.dims[a] = (int32_t)d;
-            total *= (uint64_t)g.dims[a];
+            total *= a == 2 ? (uint64_t)((g.dims[a] + kCellBand - 1) / kCellBand * kCellBand) : (uint64_t)g.dims[a];
         }
         if (ok && total <= cap) {
             g.ncells = (uint32_t)total;

@@ -22,7 +22,7 @@ struct alignas(16) SceneDev {
     float inv_cs;      // 1 / cell edge, float32 as the hash uses it
     int32_t dims[3];   // cells per axis
     uint32_t cell_base;  // global index of the scene's first cell
-    uint32_t ncells;   // dims product; global cell cell_base+ncells holds NaN positions
+    uint32_t ncells;   // dx*dy*kCellBand*ceil(dz/kCellBand); cell cell_base+ncells holds NaN positions
     uint32_t p_begin;  // first particle (AoS index == cell-sorted slot range)
     uint32_t p_end;
     uint32_t pad0;
@@ -42,6 +42,17 @@ struct alignas(16) SceneDev {
     float pad1;
 };
 static_assert(sizeof(SceneDev) == 128, "SceneDev layout");
+
+// Cells are ordered by bands of kCellBand z-planes: key of cell (x, y, z) =
+// cell_base + ((band*dy + y)*kCellBand + z%kCellBand)*dx + x, band = z/kCellBand.
+// A stencil row (fixed y, z) is still a contiguous run of cells along x.  The
+// cell-sorted slots sweep the domain band by band, each band bottom to top,
+// so the gathers' working window (the rows within reach inside one band) stays
+// small enough for an XCD's L2 on large scenes, where whole z-planes would
+// not; and the eighths of the slots that the XCDs take are still slabs of z,
+// each holding full columns of the fluid's depth.  The last band of a scene
+// may be partial; its missing planes are cells that stay empty.
+constexpr int kCellBand = 8;
 
 struct alignas(16) Tile {
     uint32_t scene;
@@ -71,12 +82,19 @@ __device__ __forceinline__ bool cell_coords(const SceneDev& sc, float x, float y
     return true;
 }
 
+// Key of cell (0, y, z): the first cell of stencil row (y, z).
+__device__ __forceinline__ uint32_t row_key(const SceneDev& sc, int y, int z)
+{
+    const uint32_t band = (uint32_t)z / (uint32_t)kCellBand, zb = (uint32_t)z % (uint32_t)kCellBand;
+    return sc.cell_base + ((band * (uint32_t)sc.dims[1] + (uint32_t)y) * (uint32_t)kCellBand + zb) * (uint32_t)sc.dims[0];
+}
+
 __device__ __forceinline__ uint32_t global_key(const SceneDev& sc, float x, float y, float z)
 {
     int cx, cy, cz;
     float fx, fy, fz;
     if (!cell_coords(sc, x, y, z, fx, fy, fz, cx, cy, cz)) return sc.cell_base + sc.ncells;
-    return sc.cell_base + ((uint32_t)cz * (uint32_t)sc.dims[1] + (uint32_t)cy) * (uint32_t)sc.dims[0] + (uint32_t)cx;
+    return row_key(sc, cy, cz) + (uint32_t)cx;
 }
 
 // integrate_comp.glsl:41-77 for one particle: a = F/rho, v' = v + dt*a,

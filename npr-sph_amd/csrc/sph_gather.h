@@ -49,7 +49,7 @@ __device__ __forceinline__ void for_each_row(const SceneDev& sc, const uint32_t*
                     x1 = (int)fminf(fmaxf(floorf(fx + w), 0.f), (float)xb);
                 }
                 if (keep) {
-                    const uint32_t row = sc.cell_base + ((uint32_t)z * (uint32_t)dy + (uint32_t)y) * (uint32_t)dx;
+                    const uint32_t row = row_key(sc, y, z);
                     b = start[row + (uint32_t)x0];
                     e = start[row + (uint32_t)(x1 + 1)];
                 }
@@ -217,7 +217,7 @@ __device__ __forceinline__ void gather_rows(const SceneDev& sc, const uint32_t* 
             const int y = ucy - SUB + k;
             rb[k] = re[k] = RB[k] = RE[k] = 0;
             if (y >= 0 && y < dy) {
-                const uint32_t row = sc.cell_base + ((uint32_t)z * (uint32_t)dy + (uint32_t)y) * (uint32_t)dx;
+                const uint32_t row = row_key(sc, y, z);
                 RB[k] = start[row + (uint32_t)xlo];
                 RE[k] = start[row + (uint32_t)xhi + 1u];
                 int x0 = xa, x1 = xb;

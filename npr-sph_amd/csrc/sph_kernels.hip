@@ -443,6 +443,84 @@ constexpr int kQueue = 16;
 constexpr int kDrainLanes = 32;
 static_assert((kQueue & (kQueue - 1)) == 0 && kQueue >= 16, "queue ring size");
 
+// One neighbour's pressure and viscosity terms for particle (xi, vi, pi),
+// accumulated into a (pressure) and b (viscosity).  The hardware reciprocal
+// square root of r^2 = 0 (a coincident pair) is +inf, so r = 0*inf is NaN, as
+// normalize(0) in the reference.
+struct PairSums {
+    float4 xi, vi;
+    float h;
+    float ax = 0.f, ay = 0.f, az = 0.f;
+    float bx = 0.f, by = 0.f, bz = 0.f;
+    __device__ __forceinline__ void add(const float4 xj, const float4 vj)
+    {
+        const float ddx = xi.x - xj.x, ddy = xi.y - xj.y, ddz = xi.z - xj.z;
+        const float r2 = fmaf(ddx, ddx, fmaf(ddy, ddy, ddz * ddz));
+        const float inv_r = __builtin_amdgcn_rsqf(r2);
+        const float r = r2 * inv_r;
+        const float q = h - r;
+        const float cp = (xi.w + xj.w) * vj.w * (q * q) * inv_r;
+        ax = fmaf(cp, ddx, ax);
+        ay = fmaf(cp, ddy, ay);
+        az = fmaf(cp, ddz, az);
+        const float cv = vj.w * q;
+        bx = fmaf(cv, vj.x - vi.x, bx);
+        by = fmaf(cv, vj.y - vi.y, by);
+        bz = fmaf(cv, vj.z - vi.z, bz);
+    }
+};
+
+// F = pres_m*a + visc_m*b + rho_i*G (force_comp.glsl:60-66), then either the
+// force into the AoS record (the force pass) or, with FUSE,
+// integrate_comp.glsl:35-81 on it: the next step's B state, its count into
+// the next grid's cells (as k_hash_count would from B) and, unless aos is
+// null, the AoS record (pos, vel, force, extras.xy).  Called by every lane
+// of the wave (the cell count takes the whole wave).
+template <bool FUSE>
+__device__ __forceinline__ void force_epilogue(const SceneDev& sc, uint32_t s, bool active, const PairSums& ps,
+                                               float rhoi, const uint32_t* __restrict__ perm,
+                                               sph_particle_t* __restrict__ aos, float4* __restrict__ PN,
+                                               float4* __restrict__ VN, uint32_t* __restrict__ key,
+                                               uint32_t* __restrict__ slot, uint32_t* __restrict__ count)
+{
+    float f[3];
+    f[0] = fmaf(sc.pres_m, ps.ax, fmaf(sc.visc_m, ps.bx, rhoi * sc.grav[0]));
+    f[1] = fmaf(sc.pres_m, ps.ay, fmaf(sc.visc_m, ps.by, rhoi * sc.grav[1]));
+    f[2] = fmaf(sc.pres_m, ps.az, fmaf(sc.visc_m, ps.bz, rhoi * sc.grav[2]));
+    if constexpr (FUSE) {
+        const float x[3] = {ps.xi.x, ps.xi.y, ps.xi.z}, v[3] = {ps.vi.x, ps.vi.y, ps.vi.z};
+        float nx[3], nv[3];
+        integrate_one(sc, x, v, f, rhoi, nx, nv);
+        uint32_t k = 0xffffffffu;
+        if (active) {
+            PN[s] = make_float4(nx[0], nx[1], nx[2], 0.f);
+            VN[s] = make_float4(nv[0], nv[1], nv[2], 0.f);
+            k = global_key(sc, nx[0], nx[1], nx[2]);
+            key[s] = k;
+        }
+        count_into_cell(k, active, s, slot, count);  // the whole wave takes part
+        if (!active || !aos) return;  // no record, or one a later step of the same call overwrites
+        float* r = reinterpret_cast<float*>(aos + perm[s]);
+        r[0] = nx[0];
+        r[1] = nx[1];
+        r[2] = nx[2];
+        r[4] = nv[0];
+        r[5] = nv[1];
+        r[6] = nv[2];
+        r[8] = f[0];
+        r[9] = f[1];
+        r[10] = f[2];
+        r[12] = rhoi;
+        r[13] = ps.xi.w;
+    } else {
+        if (!active) return;
+        float* r = reinterpret_cast<float*>(aos + perm[s]);
+        r[8] = f[0];
+        r[9] = f[1];
+        r[10] = f[2];
+    }
+}
+
 // force_comp.glsl:35-66 over the stencil, j != i (:50-53), r < h (:57):
 //   pressure  += m(p_i+p_j)/(2 rho_j) * 45/(pi h^6) * (h-r)^2 * delta/r
 //   viscosity += m(v_j-v_i)/rho_j * 45/(pi h^6) * (h-r), then * visc
@@ -480,11 +558,9 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(SUB < 3 ?
         const uint32_t si = active ? s : t.begin;
         const float4 xi = P[si];
         const float4 vi = V[si];
-        const float pi = xi.w;
         const float rhoi = rho[si];
-        const float h = sc.h, h2 = sc.h2;
-        float ax = 0.f, ay = 0.f, az = 0.f;  // pressure sum
-        float bx = 0.f, by = 0.f, bz = 0.f;  // viscosity sum
+        const float h2 = sc.h2;
+        PairSums ps{xi, vi, sc.h};
         // This lane's queue entries are [head, tail) mod kQueue, both counted
         // in bytes of a queue row (kRow per entry), so an entry's LDS address
         // is one and-or with the lane's column; head is an even entry.
@@ -493,24 +569,6 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(SUB < 3 ?
         char* const qbase = reinterpret_cast<char*>(&queue[0][0]);
         const uint32_t col = threadIdx.x * 4u;
         uint32_t head = 0, tail = 0;
-        // One neighbour's pressure and viscosity terms.  The hardware reciprocal
-        // square root of r^2 = 0 (a coincident pair) is +inf, so r = 0*inf is
-        // NaN, as normalize(0) in the reference.
-        auto pair = [&](const float4 xj, const float4 vj) {
-            const float ddx = xi.x - xj.x, ddy = xi.y - xj.y, ddz = xi.z - xj.z;
-            const float r2 = fmaf(ddx, ddx, fmaf(ddy, ddy, ddz * ddz));
-            const float inv_r = __builtin_amdgcn_rsqf(r2);
-            const float r = r2 * inv_r;
-            const float q = h - r;
-            const float cp = (pi + xj.w) * vj.w * (q * q) * inv_r;
-            ax = fmaf(cp, ddx, ax);
-            ay = fmaf(cp, ddy, ay);
-            az = fmaf(cp, ddz, az);
-            const float cv = vj.w * q;
-            bx = fmaf(cv, vj.x - vi.x, bx);
-            by = fmaf(cv, vj.y - vi.y, by);
-            bz = fmaf(cv, vj.z - vi.z, bz);
-        };
         // Processes queue entries two per round (the loads of both are issued
         // before either is used), oldest first.  With all = false it returns
         // with every lane at most kQueue - 8 entries full.  head stays even, so
@@ -528,8 +586,8 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(SUB < 3 ?
                     const uint32_t j1 = n > kRow ? e1 : sentinel;
                     const float4 x0 = f4_load(rP, j0), v0 = f4_load(rV, j0);
                     const float4 x1 = f4_load(rP, j1), v1 = f4_load(rV, j1);
-                    pair(x0, v0);
-                    pair(x1, v1);
+                    ps.add(x0, v0);
+                    ps.add(x1, v1);
                     head += 2u * kRow;
                     tail = n > kRow ? tail : head;
                 }
@@ -577,42 +635,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(SUB < 3 ?
                                     }
                                 });
         flush(true);
-        float f[3];
-        f[0] = fmaf(sc.pres_m, ax, fmaf(sc.visc_m, bx, rhoi * sc.grav[0]));
-        f[1] = fmaf(sc.pres_m, ay, fmaf(sc.visc_m, by, rhoi * sc.grav[1]));
-        f[2] = fmaf(sc.pres_m, az, fmaf(sc.visc_m, bz, rhoi * sc.grav[2]));
-        if constexpr (FUSE) {
-            const float x[3] = {xi.x, xi.y, xi.z}, v[3] = {vi.x, vi.y, vi.z};
-            float nx[3], nv[3];
-            integrate_one(sc, x, v, f, rhoi, nx, nv);
-            uint32_t k = 0xffffffffu;
-            if (active) {
-                PN[s] = make_float4(nx[0], nx[1], nx[2], 0.f);
-                VN[s] = make_float4(nv[0], nv[1], nv[2], 0.f);
-                k = global_key(sc, nx[0], nx[1], nx[2]);
-                key[s] = k;
-            }
-            count_into_cell(k, active, s, slot, count);  // the whole wave takes part
-            if (!active || !aos) return;  // no record, or one a later step of the same call overwrites
-            float* r = reinterpret_cast<float*>(aos + perm[s]);
-            r[0] = nx[0];
-            r[1] = nx[1];
-            r[2] = nx[2];
-            r[4] = nv[0];
-            r[5] = nv[1];
-            r[6] = nv[2];
-            r[8] = f[0];
-            r[9] = f[1];
-            r[10] = f[2];
-            r[12] = rhoi;
-            r[13] = pi;
-        } else {
-            if (!active) return;
-            float* r = reinterpret_cast<float*>(aos + perm[s]);
-            r[8] = f[0];
-            r[9] = f[1];
-            r[10] = f[2];
-        }
+        force_epilogue<FUSE>(sc, s, active, ps, rhoi, perm, aos, PN, VN, key, slot, count);
     });
 }
 

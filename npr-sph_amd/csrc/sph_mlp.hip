@@ -768,7 +768,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
                     const float w = __builtin_amdgcn_sqrtf(rem);
                     const int x0 = (int)fmaxf(fminf(floorf(fx - w), (float)(dx - 1)), (float)xa);
                     const int x1 = (int)fminf(fmaxf(floorf(fx + w), 0.f), (float)xb);
-                    const uint32_t row = sc.cell_base + ((uint32_t)z * (uint32_t)dy + (uint32_t)y) * (uint32_t)dx;
+                    const uint32_t row = row_key(sc, y, z);
                     rb = start[row + (uint32_t)x0];
                     rl = start[row + (uint32_t)(x1 + 1)] - rb;
                 }

@@ -60,11 +60,23 @@ float orc_smoothing_length(const orc_constants* c, const orc_fixed* f)
 
 /* ---------------------------------------------------------------- grid --- */
 
+/* Cells are ordered in bands of ORC_BAND z-planes (the solver's cell order):
+ * key(x, y, z) = ((band*dy + y)*ORC_BAND + z%ORC_BAND)*dx + x, band = z/ORC_BAND;
+ * ncells = dx*dy*ORC_BAND*ceil(dz/ORC_BAND), the last band's missing planes
+ * being empty cells. */
+#define ORC_BAND 8
+
+static inline uint32_t orc_row(const orc_grid* g, int32_t y, int32_t z)
+{
+    uint32_t band = (uint32_t)z / ORC_BAND, zb = (uint32_t)z % ORC_BAND;
+    return ((band * (uint32_t)g->dims[1] + (uint32_t)y) * ORC_BAND + zb) * (uint32_t)g->dims[0];
+}
+
 /* Cell edge = h*(1+2^-10)/sub (sub = cells per smoothing length, 1..3): a
  * particle within h of another is at most `sub` cells away on every axis
  * despite float32 rounding of the hash.  The edge grows by 2^(1/4) steps
- * until every axis has <= 2048 cells and the grid has <= max(2^19, 16n)
- * cells. */
+ * until every axis has <= 2048 cells and the banded grid has
+ * <= max(2^19, 16n) cells. */
 int orc_grid_geometry(float h, const orc_boundary* b, uint64_t n, int sub, orc_grid* g)
 {
     if (!(h > 0.0f) || !isfinite(h) || sub < 1 || sub > 3) return -1;
@@ -82,7 +94,7 @@ int orc_grid_geometry(float h, const orc_boundary* b, uint64_t n, int sub, orc_g
             if (!(d >= 1.0)) d = 1.0;
             if (d > 2048.0) { ok = 0; d = 2048.0; }
             g->dims[a] = (int32_t)d;
-            total *= (uint64_t)g->dims[a];
+            total *= a == 2 ? (uint64_t)((g->dims[a] + ORC_BAND - 1) / ORC_BAND * ORC_BAND) : (uint64_t)g->dims[a];
         }
         if (ok && total <= cap) { g->ncells = (uint32_t)total; break; }
         cs *= 1.189207115002721; /* 2^(1/4) */
@@ -95,7 +107,7 @@ int orc_grid_geometry(float h, const orc_boundary* b, uint64_t n, int sub, orc_g
 }
 
 /* float32 cell hash: coordinate c = clamp(floor((x-lo)*inv_cs), 0, dim-1),
- * key = (cz*dy + cy)*dx + cx; any NaN component gives key = ncells. */
+ * key = orc_row(cy, cz) + cx; any NaN component gives key = ncells. */
 static inline uint32_t orc_key(const orc_grid* g, float x, float y, float z)
 {
     float fx = (x - g->lo[0]) * g->inv_cs;
@@ -105,7 +117,7 @@ static inline uint32_t orc_key(const orc_grid* g, float x, float y, float z)
     int32_t cx = fx < 0.0f ? 0 : (fx >= (float)g->dims[0] ? g->dims[0] - 1 : (int32_t)fx);
     int32_t cy = fy < 0.0f ? 0 : (fy >= (float)g->dims[1] ? g->dims[1] - 1 : (int32_t)fy);
     int32_t cz = fz < 0.0f ? 0 : (fz >= (float)g->dims[2] ? g->dims[2] - 1 : (int32_t)fz);
-    return ((uint32_t)cz * (uint32_t)g->dims[1] + (uint32_t)cy) * (uint32_t)g->dims[0] + (uint32_t)cx;
+    return orc_row(g, cy, cz) + (uint32_t)cx;
 }
 
 uint32_t orc_cell_key(const orc_grid* g, float x, float y, float z) { return orc_key(g, x, y, z); }
@@ -175,16 +187,17 @@ static void orc_cells_free(orc_cells* cl)
         uint32_t key_ = (cl)->keys[i];                                                       \
         if (key_ < (cl)->g.ncells) {                                                         \
             int32_t dx_ = (cl)->g.dims[0], dy_ = (cl)->g.dims[1], dz_ = (cl)->g.dims[2];     \
+            uint32_t r_ = key_ / (uint32_t)dx_, q_ = r_ / ORC_BAND;                          \
             int32_t cx_ = (int32_t)(key_ % (uint32_t)dx_);                                   \
-            int32_t cy_ = (int32_t)((key_ / (uint32_t)dx_) % (uint32_t)dy_);                 \
-            int32_t cz_ = (int32_t)(key_ / ((uint32_t)dx_ * (uint32_t)dy_));                 \
+            int32_t cy_ = (int32_t)(q_ % (uint32_t)dy_);                                     \
+            int32_t cz_ = (int32_t)((q_ / (uint32_t)dy_) * ORC_BAND + r_ % ORC_BAND);         \
             for (int32_t z_ = cz_ - ORC_SUB; z_ <= cz_ + ORC_SUB; ++z_) {                    \
                 if (z_ < 0 || z_ >= dz_) continue;                                           \
                 for (int32_t y_ = cy_ - ORC_SUB; y_ <= cy_ + ORC_SUB; ++y_) {                \
                     if (y_ < 0 || y_ >= dy_) continue;                                       \
                     int32_t x0_ = cx_ > ORC_SUB ? cx_ - ORC_SUB : 0;                         \
                     int32_t x1_ = cx_ < dx_ - ORC_SUB ? cx_ + ORC_SUB : dx_ - 1;             \
-                    uint32_t row_ = ((uint32_t)z_ * (uint32_t)dy_ + (uint32_t)y_) * (uint32_t)dx_; \
+                    uint32_t row_ = orc_row(&(cl)->g, y_, z_);                              \
                     for (uint32_t s_ = (cl)->start[row_ + x0_]; s_ < (cl)->start[row_ + x1_ + 1]; ++s_) { \
                         uint64_t J = (cl)->perm[s_];                                         \
                         __VA_ARGS__                                                          \

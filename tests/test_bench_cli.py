@@ -33,18 +33,19 @@ def test_bad_step_counts_are_refused_before_any_gpu_work():
 
 def test_committed_profile_records_are_keyed_and_sourced():
     """The bench line's roofline.traffic and counter-derived utilisations come
-    from profiles/traffic_<workload>.json; a record applies only to the
+    from profiles/traffic_<workload>_<tuning tag>.json; a record applies only to the
     kernel, settled state and tuning it was measured at, and names the
     committed rocprofv3 summary it was computed from."""
     import glob
     bench = _bench()
     recs = sorted(glob.glob(os.path.join(ROOT, "profiles", "traffic_*.json")))
+    assert recs
     for path in recs:
         rec = json.load(open(path))
         for key in ("kernel", "workload", "settle", "tuning", "traffic_bytes_per_launch", "source",
                     "rocprof_avg_ms_timed_launches", "valu_busy"):
             assert key in rec, (path, key)
-        assert os.path.basename(path) == f"traffic_{rec['workload']}.json"
+        assert os.path.basename(path) == f"traffic_{rec['workload']}_{bench.profile_tag(rec['tuning'])}.json"
         assert rec["traffic_bytes_per_launch"] > 0
         assert 0.0 < rec["valu_busy"] <= 1.0
         assert os.path.exists(os.path.join(ROOT, rec["source"]))

@@ -246,7 +246,8 @@ def _numpy_keys(pos, g):
     nan = np.isnan(f).any(axis=1)
     c = np.where(f < 0, 0, np.where(f >= dims.astype(np.float32), dims - 1,
                                      np.trunc(np.nan_to_num(f, nan=0.0, posinf=0.0, neginf=0.0)))).astype(np.int64)
-    key = (c[:, 2] * dims[1] + c[:, 1]) * dims[0] + c[:, 0]
+    band, zb = c[:, 2] // 8, c[:, 2] % 8  # bands of 8 z-planes
+    key = ((band * dims[1] + c[:, 1]) * 8 + zb) * dims[0] + c[:, 0]
     return np.where(nan, g.ncells, key).astype(np.uint32)
 
 
@@ -285,6 +286,9 @@ def test_grid_geometry_rule(orc):
     big = orc.make_boundary((-100, -100, -100), (100, 100, 100))
     g2 = orc.grid_geometry(h, big, 1000)
     assert max(g2.dims) <= 2048 and g2.ncells <= 2 ** 19
+    # cells are counted in whole bands of 8 z-planes (the last band padded)
+    for gg in (g, g1, g2, g3):
+        assert gg.ncells == gg.dims[0] * gg.dims[1] * 8 * -(-gg.dims[2] // 8)
     # neighbour safety: cell edge * 2 >= h always
     assert 2 * g2.cs >= float(h)
 

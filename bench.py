@@ -313,6 +313,8 @@ def profile_record(workload, kernel, tuning, settle):
 # The library keys cells in bands of this many z-planes:
 # key = ((z / band * dy + y) * band + z % band) * dx + x.
 CELL_BAND = 8
+# Its cells are this many times narrower along x than along y and z.
+X_SPLIT = 2
 
 # Useful FP32 flops per particle of the gathers (the flop model of the
 # three-pass path): C candidates scanned, K neighbours within h.
@@ -336,14 +338,16 @@ def gather_counts(sim, state, scene=0):
     pos = state["pos"][first:first + count, :3].astype(np.float32)
     lo = np.array(gi.lo[:3], dtype=np.float32)
     dims = np.array(gi.dims[:3])
-    f = (pos - lo) * np.float32(gi.inv_cell)
+    inv = np.array([np.float32(X_SPLIT) * np.float32(gi.inv_cell), gi.inv_cell, gi.inv_cell], dtype=np.float32)
+    f = (pos - lo) * inv
     ok = ~np.isnan(f).any(axis=1)
     f = np.where(ok[:, None], f, 0.0).astype(np.float32)
     c = np.clip(np.floor(f), 0, dims - 1).astype(np.int64)
     c = np.where(f < 0, 0, np.where(f >= dims.astype(np.float32), dims - 1, c))
     dx, dy, dz = (int(v) for v in dims)
-    xa = np.maximum(c[:, 0] - sub, 0)
-    xb = np.minimum(c[:, 0] + sub, dx - 1)
+    subx = X_SPLIT * sub
+    xa = np.maximum(c[:, 0] - subx, 0)
+    xb = np.minimum(c[:, 0] + subx, dx - 1)
     total = np.zeros(count, dtype=np.int64)
     base = gi.cell_base
     for oz in range(-sub, sub + 1):
@@ -357,7 +361,7 @@ def gather_counts(sim, state, scene=0):
                 gy = np.where(y > c[:, 1], y - f[:, 1], np.where(y < c[:, 1], f[:, 1] - (y + 1), 0.0))
                 rem = sub * sub - (gy * gy + gz * gz)
                 inb &= rem > 0
-                w = np.sqrt(np.maximum(rem, 0.0))
+                w = np.float32(X_SPLIT) * np.sqrt(np.maximum(rem, 0.0)).astype(np.float32)
                 x0 = np.maximum(np.minimum(np.floor(f[:, 0] - w), dx - 1), xa).astype(np.int64)
                 x1 = np.minimum(np.maximum(np.floor(f[:, 0] + w), 0), xb).astype(np.int64)
             zc, yc = np.clip(z, 0, dz - 1), np.clip(y, 0, dy - 1)

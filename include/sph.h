@@ -95,11 +95,18 @@ typedef struct sph_lattice {
     uint64_t seed;
 } sph_lattice_t;
 
-/* Geometry of one scene's uniform neighbour grid (read-only diagnostics). */
+/* Cells are SPH_GRID_X_SPLIT times narrower along x than along y and z. */
+#define SPH_GRID_X_SPLIT 2
+
+/* Geometry of one scene's uniform neighbour grid (read-only diagnostics).
+ * Cell coordinate along an axis = clamp(floor((p - lo) * inv), 0, dims - 1) in
+ * float32, inv = SPH_GRID_X_SPLIT * inv_cell (exact) along x, inv_cell along
+ * y and z. */
 typedef struct sph_grid_info {
     float lo[3];          /* grid origin = boundary lower corner */
-    float inv_cell;       /* 1 / cell edge (float32, used by the cell hash) */
-    double cell;          /* cell edge (>= smoothing length / cells_per_h) */
+    float inv_cell;       /* 1 / cell edge along y and z (float32, used by the cell hash) */
+    double cell;          /* cell edge along y and z (>= smoothing length / cells_per_h);
+                             cell / SPH_GRID_X_SPLIT along x */
     int32_t dims[3];      /* cells per axis */
     uint32_t ncells;      /* dims[0]*dims[1]*8*ceil(dims[2]/8): cells are keyed band by band,
                              key = ((z/8*dims[1] + y)*8 + z%8)*dims[0] + x; cell `ncells`
@@ -116,7 +123,8 @@ typedef struct sph_grid_info {
  * blocks_per_xcd never change the bits (only cells_per_h changes the
  * neighbour summation order).  Defaults: {2, 1, 128, 0}. */
 typedef struct sph_tuning {
-    int32_t cells_per_h;     /* 1..3: cell edge = h*(1+2^-10)/cells_per_h */
+    int32_t cells_per_h;     /* 1..3: cell edge = h*(1+2^-10)/cells_per_h along y and z,
+                                SPH_GRID_X_SPLIT times narrower along x */
     int32_t prune;           /* 1: skip stencil rows and cells farther than h from the particle */
     int32_t block;           /* 128 or 256 particles per gather workgroup */
     int32_t blocks_per_xcd;  /* 0: one workgroup per tile; >0: persistent workgroups per XCD */

@@ -39,23 +39,24 @@ struct GridGeom {
     uint32_t ncells;
 };
 
-// Cell edge h*(1+2^-10)/sub: with float32 rounding of the hash, |xi-xj| < h
-// still lands within `sub` cells on every axis.  Grown by 2^(1/4) until each
-// axis has <= 2048 cells and the grid, z rounded up to whole bands of
-// kCellBand planes, has <= max(2^19, 16n) cells.
+// Cell edge cs = h*(1+2^-10)/sub along y and z and cs/kXSplit along x: with
+// float32 rounding of the hash, |xi-xj| < h still lands within `sub` cells
+// along y and z and kXSplit*sub along x.  Grown by 2^(1/4) until each axis has
+// <= 2048 cells and the grid, z rounded up to whole bands of kCellBand planes,
+// has <= kXSplit * max(2^19, 16n) cells.
 bool grid_geometry(float h, const sph_boundary_t& b, uint64_t n, int sub, GridGeom& g)
 {
     if (!(h > 0.0f) || !std::isfinite(h) || sub < 1 || sub > 3) return false;
     for (int a = 0; a < 3; ++a)
         if (!std::isfinite(b.lower[a]) || !std::isfinite(b.upper[a])) return false;
     double cs = (double)h * (1.0 + 1.0 / 1024.0) / sub;
-    const uint64_t cap = std::max<uint64_t>(524288u, 16u * n);
+    const uint64_t cap = (uint64_t)kXSplit * std::max<uint64_t>(524288u, 16u * n);
     for (;;) {
         uint64_t total = 1;
         bool ok = true;
         for (int a = 0; a < 3; ++a) {
             const double ext = (double)b.upper[a] - (double)b.lower[a];
-            double d = std::ceil(ext / cs);
+            double d = std::ceil(ext / (a == 0 ? cs / kXSplit : cs));
             if (!(d >= 1.0)) d = 1.0;
             if (d > 2048.0) {
                 ok = false;
@@ -222,6 +223,7 @@ SceneDev make_scene(const sph_constants_t& c, const sph_boundary_t& b, const sph
         s.upper[a] = b.upper[a];
     }
     s.inv_cs = g.inv_cs;
+    s.inv_csx = (float)kXSplit * g.inv_cs;  // exact: a power of two
     s.cell_base = cell_base;
     s.ncells = g.ncells;
     s.p_begin = p_begin;

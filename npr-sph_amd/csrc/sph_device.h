@@ -19,13 +19,13 @@ constexpr int kTile = 256;
 // parameter change is one small device write and needs no graph re-capture.
 struct alignas(16) SceneDev {
     float lo[3];       // grid origin (boundary lower corner)
-    float inv_cs;      // 1 / cell edge, float32 as the hash uses it
+    float inv_cs;      // 1 / cell edge along y and z, float32 as the hash uses it
     int32_t dims[3];   // cells per axis
     uint32_t cell_base;  // global index of the scene's first cell
     uint32_t ncells;   // dx*dy*kCellBand*ceil(dz/kCellBand); cell cell_base+ncells holds NaN positions
     uint32_t p_begin;  // first particle (AoS index == cell-sorted slot range)
     uint32_t p_end;
-    uint32_t pad0;
+    float inv_csx;     // 1 / cell edge along x = kXSplit * inv_cs (exact)
     float h;           // smoothing length, smoothing_coeff*PARTICLE_RADIUS (rho_pres_comp.glsl:40)
     float h2;
     float poly6_m;     // mass*315/(64*pi*h^9)                 (rho_pres_comp.glsl:52)
@@ -54,6 +54,13 @@ static_assert(sizeof(SceneDev) == 128, "SceneDev layout");
 // may be partial; its missing planes are cells that stay empty.
 constexpr int kCellBand = 8;
 
+// Cells are kXSplit times narrower along x than along y and z.  A stencil row
+// (fixed y, z) is a contiguous slot range whatever the x edge, so narrower x
+// cells trim the chord ends of each row (fewer candidates) without adding
+// rows; only the cell count grows.  A particle's stencil reaches kXSplit*sub
+// cells along x and sub cells along y and z.
+constexpr int kXSplit = 2;
+
 struct alignas(16) Tile {
     uint32_t scene;
     uint32_t begin;  // first particle / slot of the tile
@@ -61,18 +68,19 @@ struct alignas(16) Tile {
     uint32_t pad;
 };
 
-// Cell coordinates of a position: c = clamp(floor((x-lo)*inv_cs), 0, dim-1).
-// Float32 with no contraction, so the CPU build reproduces it bit for bit.
-// Returns false when any component is NaN (such particles go to the scene's
-// NaN cell, which is in no stencil: every comparison with NaN is false in the
-// reference as well).  With a cell edge >= h/sub, clamping keeps
-// |xi-xj| < h => |ci-cj| <= sub on every axis.
+// Cell coordinates of a position: c = clamp(floor((x-lo)*inv), 0, dim-1), inv
+// = inv_csx along x and inv_cs along y and z.  Float32 with no contraction, so
+// the CPU build reproduces it bit for bit.  Returns false when any component is
+// NaN (such particles go to the scene's NaN cell, which is in no stencil: every
+// comparison with NaN is false in the reference as well).  With edges >= h/sub
+// (y, z) and h/(kXSplit*sub) (x), clamping keeps |xi-xj| < h => |ci-cj| <= sub
+// along y and z and <= kXSplit*sub along x.
 // f* receive the unclamped fractional cell coordinates.
 __device__ __forceinline__ bool cell_coords(const SceneDev& sc, float x, float y, float z, float& fx, float& fy,
                                             float& fz, int& cx, int& cy, int& cz)
 {
 #pragma clang fp contract(off)
-    fx = (x - sc.lo[0]) * sc.inv_cs;
+    fx = (x - sc.lo[0]) * sc.inv_csx;
     fy = (y - sc.lo[1]) * sc.inv_cs;
     fz = (z - sc.lo[2]) * sc.inv_cs;
     if (fx != fx || fy != fy || fz != fz) return false;

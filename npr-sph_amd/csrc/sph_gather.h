@@ -8,9 +8,11 @@ namespace nprsph {
 
 // Visits the stencil rows of a particle at fractional cell coordinates
 // (fx,fy,fz) in cell (cx,cy,cz): rows (y,z) within SUB cells, each a slot
-// range [b,e) covering x-adjacent cells.  With PRUNE, rows whose y/z gap to
-// the particle is >= SUB cells are skipped and the x extent is cut to
-// floor(fx -+ w), w = sqrt(SUB^2 - gap^2).  SUB cells exceed h by the factor
+// range [b,e) covering the x cells within SUBX = kXSplit*SUB.  With PRUNE,
+// rows whose y/z gap to the particle is >= SUB cells are skipped and the x
+// extent is cut to floor(fx -+ kXSplit*w), w = sqrt(SUB^2 - gap^2) in y/z
+// cell edges (kXSplit*w in the kXSplit times narrower x cells; the factor is
+// exact).  SUB y/z cells and SUBX x cells exceed h by the factor
 // 1+2^-10, which covers the float32 rounding of the gaps and of the hardware
 // square root, so pruning only drops pairs with r >= h.  Border cells also
 // hold clamped particles beyond the box: gaps measured from the unclamped
@@ -24,9 +26,10 @@ __device__ __forceinline__ void for_each_row(const SceneDev& sc, const uint32_t*
                                              float fz, int cx, int cy, int cz, uint32_t* rows, Body&& body)
 {
     constexpr int kRows = 2 * SUB + 1;
+    constexpr int SUBX = kXSplit * SUB;
     const int dx = sc.dims[0], dy = sc.dims[1], dz = sc.dims[2];
-    const int xa = cx > SUB ? cx - SUB : 0;
-    const int xb = cx < dx - SUB ? cx + SUB : dx - 1;
+    const int xa = cx > SUBX ? cx - SUBX : 0;
+    const int xb = cx < dx - SUBX ? cx + SUBX : dx - 1;
     const int z0 = cz > SUB ? cz - SUB : 0, z1 = cz < dz - SUB ? cz + SUB : dz - 1;
     constexpr float kReach2 = (float)(SUB * SUB);
     for (int z = z0; z <= z1; ++z) {
@@ -42,7 +45,7 @@ __device__ __forceinline__ void for_each_row(const SceneDev& sc, const uint32_t*
                     const float gy = y > cy ? (float)y - fy : (y < cy ? fy - (float)(y + 1) : 0.f);
                     const float rem = kReach2 - fmaf(gy, gy, gz * gz);
                     keep = rem > 0.f;
-                    const float w = __builtin_amdgcn_sqrtf(fmaxf(rem, 0.f));
+                    const float w = (float)kXSplit * __builtin_amdgcn_sqrtf(fmaxf(rem, 0.f));
                     // Cells 0 and dx-1 extend to -inf/+inf, so the window ends
                     // are clamped to them before meeting the stencil bounds.
                     x0 = (int)fmaxf(fminf(floorf(fx - w), (float)(dx - 1)), (float)xa);
@@ -147,7 +150,7 @@ struct StageRow {
 // Stencil walk of one wave.  When every valid lane of the wave lies in the
 // same cell row (cy, cz) - the common case, since slots are cell-sorted x
 // fastest - the slot range of each stencil row over the wave's x cells
-// widened by SUB is contiguous.  That range is copied into the wave's LDS
+// widened by SUBX is contiguous.  That range is copied into the wave's LDS
 // stage kStage candidates per round, lane k loading slots k, k+64, ...
 // (coalesced), and each lane then reads its own pruned sub-range of the round
 // from LDS.  With PREFETCH the first round of row k+1 is loaded into
@@ -168,6 +171,7 @@ __device__ __forceinline__ void gather_rows(const SceneDev& sc, const uint32_t* 
                                             Body&& body)
 {
     constexpr int kRows = 2 * SUB + 1;
+    constexpr int SUBX = kXSplit * SUB;
     constexpr int kPer = (kStage + 63) / 64;
     const int lane = (int)__lane_id();
     const int dx = sc.dims[0], dy = sc.dims[1], dz = sc.dims[2];
@@ -184,8 +188,8 @@ __device__ __forceinline__ void gather_rows(const SceneDev& sc, const uint32_t* 
     // Wave-uniform row and x extent.
     const int row_yz = __builtin_amdgcn_readfirstlane(rmin);
     const int ucy = row_yz % dy, ucz = row_yz / dy;
-    const int xlo = __builtin_amdgcn_readfirstlane(max(wave_min_i32(valid ? cx : 0x7fffffff) - SUB, 0));
-    const int xhi = __builtin_amdgcn_readfirstlane(min(wave_max_i32(valid ? cx : -1) + SUB, dx - 1));
+    const int xlo = __builtin_amdgcn_readfirstlane(max(wave_min_i32(valid ? cx : 0x7fffffff) - SUBX, 0));
+    const int xhi = __builtin_amdgcn_readfirstlane(min(wave_max_i32(valid ? cx : -1) + SUBX, dx - 1));
     float* stage = reinterpret_cast<float*>(lds);
     auto fetch = [&](float4 (&dst)[kPer], uint32_t c0, uint32_t n) {
 #pragma unroll
@@ -205,8 +209,8 @@ __device__ __forceinline__ void gather_rows(const SceneDev& sc, const uint32_t* 
             }
         }
     };
-    const int xa = cx > SUB ? cx - SUB : 0;
-    const int xb = cx < dx - SUB ? cx + SUB : dx - 1;
+    const int xa = cx > SUBX ? cx - SUBX : 0;
+    const int xb = cx < dx - SUBX ? cx + SUBX : dx - 1;
     const int z0 = ucz > SUB ? ucz - SUB : 0, z1 = ucz < dz - SUB ? ucz + SUB : dz - 1;
     constexpr float kReach2 = (float)(SUB * SUB);
     for (int z = z0; z <= z1; ++z) {
@@ -226,7 +230,7 @@ __device__ __forceinline__ void gather_rows(const SceneDev& sc, const uint32_t* 
                     const float gy = y > cy ? (float)y - fy : (y < cy ? fy - (float)(y + 1) : 0.f);
                     const float rem = kReach2 - fmaf(gy, gy, gz * gz);
                     keep = keep && rem > 0.f;
-                    const float w = __builtin_amdgcn_sqrtf(fmaxf(rem, 0.f));
+                    const float w = (float)kXSplit * __builtin_amdgcn_sqrtf(fmaxf(rem, 0.f));
                     x0 = (int)fmaxf(fminf(floorf(fx - w), (float)(dx - 1)), (float)xa);
                     x1 = (int)fminf(fmaxf(floorf(fx + w), 0.f), (float)xb);
                 }

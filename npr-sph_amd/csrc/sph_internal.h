@@ -16,9 +16,9 @@ constexpr int kScanItems = 8;
 constexpr int kScanBlock = kScanThreads * kScanItems;  // cells per scan block
 
 inline int scan_blocks(uint32_t ncells) { return (int)((ncells + kScanBlock - 1) / kScanBlock); }
-// Grids of at most this many scan blocks skip k_scan_top: k_scan_down sums
-// the block totals before each block itself.
-constexpr int kScanFoldBlocks = 2048;
+// Grids of at most this many scan blocks (8 M cells) skip k_scan_top:
+// k_scan_down sums the block totals before each block itself.
+constexpr int kScanFoldBlocks = 4096;
 inline bool scan_folds(uint32_t ncells) { return scan_blocks(ncells) <= kScanFoldBlocks; }
 
 // Device buffers of one solver instance.  "A" is the cell-sorted state the

@@ -749,7 +749,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
         float fx, fy, fz;
         if (cell_coords(sc, xi.x, xi.y, xi.z, fx, fy, fz, cx, cy, cz)) {
             const int dx = sc.dims[0], dy = sc.dims[1], dz = sc.dims[2];
-            const int xa = cx > SUB ? cx - SUB : 0, xb = cx < dx - SUB ? cx + SUB : dx - 1;
+            constexpr int SUBX = kXSplit * SUB;
+            const int xa = cx > SUBX ? cx - SUBX : 0, xb = cx < dx - SUBX ? cx + SUBX : dx - 1;
             const int z0 = cz > SUB ? cz - SUB : 0, z1 = cz < dz - SUB ? cz + SUB : dz - 1;
             const int y0 = cy > SUB ? cy - SUB : 0, y1 = cy < dy - SUB ? cy + SUB : dy - 1;
             constexpr float kReach2 = (float)(SUB * SUB);
@@ -765,7 +766,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
                 const float gy = y > cy ? (float)y - fy : (y < cy ? fy - (float)(y + 1) : 0.f);
                 const float rem = kReach2 - fmaf(gy, gy, gz * gz);
                 if (rem > 0.f) {
-                    const float w = __builtin_amdgcn_sqrtf(rem);
+                    const float w = (float)kXSplit * __builtin_amdgcn_sqrtf(rem);
                     const int x0 = (int)fmaxf(fminf(floorf(fx - w), (float)(dx - 1)), (float)xa);
                     const int x1 = (int)fminf(fmaxf(floorf(fx + w), 0.f), (float)xb);
                     const uint32_t row = row_key(sc, y, z);

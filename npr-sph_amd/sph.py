@@ -104,6 +104,12 @@ class Lattice(C.Structure):
                 ("origin", C.c_float * 3), ("spacing", C.c_float), ("jitter", C.c_float), ("seed", C.c_uint64)]
 
 
+# Cells are this many times narrower along x than along y and z
+# (SPH_GRID_X_SPLIT in include/sph.h): the x coordinate of a cell uses
+# GRID_X_SPLIT * inv_cell.
+GRID_X_SPLIT = 2
+
+
 class GridInfo(C.Structure):
     _fields_ = [("lo", C.c_float * 3), ("inv_cell", C.c_float), ("cell", C.c_double), ("dims", C.c_int32 * 3),
                 ("ncells", C.c_uint32), ("cell_base", C.c_uint32), ("first", C.c_uint32), ("count", C.c_uint32),

@@ -72,11 +72,16 @@ static inline uint32_t orc_row(const orc_grid* g, int32_t y, int32_t z)
     return ((band * (uint32_t)g->dims[1] + (uint32_t)y) * ORC_BAND + zb) * (uint32_t)g->dims[0];
 }
 
-/* Cell edge = h*(1+2^-10)/sub (sub = cells per smoothing length, 1..3): a
- * particle within h of another is at most `sub` cells away on every axis
- * despite float32 rounding of the hash.  The edge grows by 2^(1/4) steps
- * until every axis has <= 2048 cells and the banded grid has
- * <= max(2^19, 16n) cells. */
+/* Cells are ORC_XSPLIT times narrower along x than along y and z (the
+ * solver's cells). */
+#define ORC_XSPLIT 2
+
+/* Cell edge cs = h*(1+2^-10)/sub along y and z (sub = cells per smoothing
+ * length, 1..3) and cs/ORC_XSPLIT along x: a particle within h of another is
+ * at most `sub` cells away along y and z and ORC_XSPLIT*sub along x despite
+ * float32 rounding of the hash.  The edge grows by 2^(1/4) steps until every
+ * axis has <= 2048 cells and the banded grid has <= ORC_XSPLIT*max(2^19, 16n)
+ * cells. */
 int orc_grid_geometry(float h, const orc_boundary* b, uint64_t n, int sub, orc_grid* g)
 {
     if (!(h > 0.0f) || !isfinite(h) || sub < 1 || sub > 3) return -1;
@@ -85,12 +90,13 @@ int orc_grid_geometry(float h, const orc_boundary* b, uint64_t n, int sub, orc_g
     double cs = (double)h * (1.0 + 1.0 / 1024.0) / sub;
     uint64_t cap = 16u * n;
     if (cap < 524288u) cap = 524288u;
+    cap *= ORC_XSPLIT;
     for (;;) {
         uint64_t total = 1;
         int ok = 1;
         for (int a = 0; a < 3; ++a) {
             double ext = (double)b->upper[a] - (double)b->lower[a];
-            double d = ceil(ext / cs);
+            double d = ceil(ext / (a == 0 ? cs / ORC_XSPLIT : cs));
             if (!(d >= 1.0)) d = 1.0;
             if (d > 2048.0) { ok = 0; d = 2048.0; }
             g->dims[a] = (int32_t)d;
@@ -106,11 +112,13 @@ int orc_grid_geometry(float h, const orc_boundary* b, uint64_t n, int sub, orc_g
     return 0;
 }
 
-/* float32 cell hash: coordinate c = clamp(floor((x-lo)*inv_cs), 0, dim-1),
+/* float32 cell hash: coordinate c = clamp(floor((x-lo)*inv), 0, dim-1) with
+ * inv = ORC_XSPLIT*inv_cs (exact) along x and inv_cs along y and z,
  * key = orc_row(cy, cz) + cx; any NaN component gives key = ncells. */
 static inline uint32_t orc_key(const orc_grid* g, float x, float y, float z)
 {
-    float fx = (x - g->lo[0]) * g->inv_cs;
+    const float inv_x = (float)ORC_XSPLIT * g->inv_cs;
+    float fx = (x - g->lo[0]) * inv_x;
     float fy = (y - g->lo[1]) * g->inv_cs;
     float fz = (z - g->lo[2]) * g->inv_cs;
     if (fx != fx || fy != fy || fz != fz) return g->ncells;
@@ -160,6 +168,7 @@ typedef struct {
 /* The oracle's own neighbour walk uses 2 cells per h; any value gives the same
  * neighbour sets. */
 #define ORC_SUB 2
+#define ORC_SUBX (ORC_XSPLIT * ORC_SUB)
 static int orc_cells_build(orc_cells* cl, uint64_t n, const double* x, float h, const orc_boundary* b)
 {
     if (orc_grid_geometry(h, b, n, ORC_SUB, &cl->g) != 0) return -1;
@@ -180,7 +189,8 @@ static void orc_cells_free(orc_cells* cl)
     free(cl->perm);
 }
 
-/* Visit the (2*ORC_SUB+1)^3 cell stencil around particle i's cell.  The NaN cell has no
+/* Visit the cell stencil around particle i's cell: ORC_SUB cells along y and
+ * z, ORC_SUBX along x.  The NaN cell has no
  * neighbours, as in the reference, where every comparison with NaN is false. */
 #define ORC_FOR_NEIGHBOURS(cl, i, J, ...)                                                    \
     do {                                                                                     \
@@ -195,8 +205,8 @@ static void orc_cells_free(orc_cells* cl)
                 if (z_ < 0 || z_ >= dz_) continue;                                           \
                 for (int32_t y_ = cy_ - ORC_SUB; y_ <= cy_ + ORC_SUB; ++y_) {                \
                     if (y_ < 0 || y_ >= dy_) continue;                                       \
-                    int32_t x0_ = cx_ > ORC_SUB ? cx_ - ORC_SUB : 0;                         \
-                    int32_t x1_ = cx_ < dx_ - ORC_SUB ? cx_ + ORC_SUB : dx_ - 1;             \
+                    int32_t x0_ = cx_ > ORC_SUBX ? cx_ - ORC_SUBX : 0;                       \
+                    int32_t x1_ = cx_ < dx_ - ORC_SUBX ? cx_ + ORC_SUBX : dx_ - 1;           \
                     uint32_t row_ = orc_row(&(cl)->g, y_, z_);                              \
                     for (uint32_t s_ = (cl)->start[row_ + x0_]; s_ < (cl)->start[row_ + x1_ + 1]; ++s_) { \
                         uint64_t J = (cl)->perm[s_];                                         \
@@ -237,7 +247,8 @@ void orc_density_bf(uint64_t n, const double* x, const orc_constants* c, const o
 }
 
 /* Same sums restricted to the cell stencil of a cell list built from the
- * float32 image of x; ORC_SUB cells span >= h, so the neighbour set is exact. */
+ * float32 image of x; ORC_SUB y/z cells and ORC_SUBX x cells span >= h, so
+ * the neighbour set is exact. */
 int orc_density_grid(uint64_t n, const double* x, const orc_constants* c, const orc_fixed* f,
                      const orc_boundary* b, double* rho, double* p)
 {

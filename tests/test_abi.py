@@ -51,6 +51,16 @@ def test_struct_layouts(pkg):
     assert s.PARTICLE_DTYPE.fields["pos"][1] == 0 and s.PARTICLE_DTYPE.fields["extras"][1] == 48
 
 
+def test_grid_x_split_constant_agrees(pkg):
+    """The x split of the cell grid is declared once in the header; the Python
+    binding and the bench's candidate counter use the same value."""
+    hdr = open(os.path.join(ROOT, "include", "sph.h")).read()
+    m = re.search(r"#define SPH_GRID_X_SPLIT (\d+)", hdr)
+    assert m and int(m.group(1)) == pkg.sph.GRID_X_SPLIT
+    bench = open(os.path.join(ROOT, "bench.py")).read()
+    assert re.search(r"^X_SPLIT = (\d+)$", bench, re.M).group(1) == m.group(1)
+
+
 def test_defaults_are_the_references(pkg, gpu_lib):
     c, b, f = pkg.sph.defaults()
     assert (c.mass, c.smoothing_coeff, c.visc, c.resting_rho) == (np.float32(0.02), 4.0, 3000.0, 1000.0)

@@ -12,6 +12,7 @@ Each check is chosen so that a plausible transcription mistake fails it:
     numpy implementation, the lattice generator against pure Python.
 """
 import json
+import math
 import os
 
 import numpy as np
@@ -240,7 +241,8 @@ def test_nan_particle_is_inert(orc):
 def _numpy_keys(pos, g):
     """Independent float32 cell hash (numpy scalar semantics, no FMA)."""
     lo = np.array(g.lo[:3], dtype=np.float32)
-    inv = np.float32(g.inv_cs)
+    # x cells are half as wide as y/z cells: 2 * inv_cs along x (exact)
+    inv = np.array([np.float32(2) * np.float32(g.inv_cs), g.inv_cs, g.inv_cs], dtype=np.float32)
     dims = np.array(g.dims[:3])
     f = (pos.astype(np.float32) - lo) * inv
     nan = np.isnan(f).any(axis=1)
@@ -272,25 +274,31 @@ def test_grid_build_matches_numpy(orc, n):
 
 
 def test_grid_geometry_rule(orc):
-    """Cell edge h(1+2^-10)/sub; dims cover the box; caps hold."""
+    """Cell edge h(1+2^-10)/sub along y and z, half that along x; dims cover
+    the box; the grid holds at most 2 max(2^19, 16n) cells."""
     h = orc.smoothing_length()
     g = orc.grid_geometry(h, orc.default_boundary(), 10000)
     assert g.cs == pytest.approx(float(h) * (1 + 2 ** -10) / 2, rel=1e-15)
-    assert tuple(g.dims) == (60, 135, 60)
+    assert tuple(g.dims) == (120, 135, 60)  # box 0.6 x 1.35 x 0.6, x cells of cs/2
     g1 = orc.grid_geometry(h, orc.default_boundary(), 10000, sub=1)
-    assert tuple(g1.dims) == (30, 68, 30) and g1.sub == 1
+    assert tuple(g1.dims) == (60, 68, 30) and g1.sub == 1
     g3 = orc.grid_geometry(h, orc.default_boundary(), 10 ** 6, sub=3)
     assert g3.cs == pytest.approx(float(h) * (1 + 2 ** -10) / 3, rel=1e-15) and g3.sub == 3
     with pytest.raises(ValueError):
         orc.grid_geometry(h, orc.default_boundary(), 10, sub=4)
     big = orc.make_boundary((-100, -100, -100), (100, 100, 100))
     g2 = orc.grid_geometry(h, big, 1000)
-    assert max(g2.dims) <= 2048 and g2.ncells <= 2 ** 19
+    assert max(g2.dims) <= 2048 and g2.ncells <= 2 ** 20
     # cells are counted in whole bands of 8 z-planes (the last band padded)
     for gg in (g, g1, g2, g3):
         assert gg.ncells == gg.dims[0] * gg.dims[1] * 8 * -(-gg.dims[2] // 8)
     # neighbour safety: cell edge * 2 >= h always
     assert 2 * g2.cs >= float(h)
+    # every axis covers the box: x in cells of cs/2, y and z in cells of cs
+    for gg, bb in ((g, orc.default_boundary()), (g2, big)):
+        ext = [float(bb.upper[a]) - float(bb.lower[a]) for a in range(3)]
+        assert gg.dims[0] == min(2048, max(1, math.ceil(ext[0] / (gg.cs / 2))))
+        assert gg.dims[1] == min(2048, max(1, math.ceil(ext[1] / gg.cs)))
 
 
 # --------------------------------------------------------- generators ---

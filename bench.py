@@ -72,12 +72,12 @@ LEARNED_WINDOW = 10
 MLP_FLOPS_PER_PAIR = 2 * (9 * 64 + 64 * 64 + 64 * 3)
 # Neighbour build (no reference counterpart), compulsory bytes per particle of
 # the kernels each step launches (the cell key and count of every particle are
-# taken by the previous step's fused force kernel; k_hash_count runs only
-# after an upload):
+# taken by the previous step's fused force kernel, with the scan's block sums;
+# k_hash_count runs only after an upload):
 #   scatter       key, rank, cell start, index 16 read; 3 index arrays 12 written = 28
 #   rank_reorder  3 index arrays 12 + pos, vel 24 read; index 4 + pos, vel 24 written = 64
 # (the per-cell count/scan arrays are per cell, not per particle, and left out)
-BUILD_KERNELS = ("scan_reduce", "scan_top", "scan_down", "scatter", "rank_reorder")
+BUILD_KERNELS = ("scan_top", "scan_down", "scatter", "rank_reorder")
 BUILD_BYTES = 28 + 64
 BF16_DENSE_TFLOPS = 2500.0
 

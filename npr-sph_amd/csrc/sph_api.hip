@@ -163,6 +163,10 @@ struct sph_ctx {
     // count[] may hold counts no build will consume (the state was dropped or
     // the grid changed after a counting step): cleared before the next count.
     bool counts_dirty = false;
+    // bsum holds the scan's block sums of count[]: both cleared by a build,
+    // then counted together by the fused step.  A count by k_hash_count
+    // leaves bsum behind, and the next build first runs k_scan_reduce.
+    bool bsum_valid = true;
 
     hipGraphExec_t graph[2] = {nullptr, nullptr};  // 1 step, kUnroll steps
     hipEvent_t inflight[kMaxGraphsInFlight] = {};  // completion of recent graph launches
@@ -315,9 +319,8 @@ int apply_params(sph_ctx* c, bool geometry_may_change)
             return fail(c, SPH_ERR_NOMEM, "cannot allocate grid cells");
         }
         c->cell_cap = cap;
-        hipError_t e = launch_fill_u32(c->stream, c->d.count, cap, 0u);
-        if (e != hipSuccess) return hip_fail(c, e, "clear cell counts");
-        c->counted = c->counts_dirty = false;
+        c->counted = false;
+        c->counts_dirty = true;  // cleared (with the block sums) before the next count
     }
     const int nb = scan_blocks(cells);
     if (nb > c->bsum_cap) {
@@ -330,6 +333,8 @@ int apply_params(sph_ctx* c, bool geometry_may_change)
             return fail(c, SPH_ERR_NOMEM, "cannot allocate scan scratch");
         }
         c->bsum_cap = nb;
+        c->counted = false;
+        c->counts_dirty = true;
     }
 
     uint32_t base = 0;
@@ -397,10 +402,22 @@ int ensure_counted(sph_ctx* c)
     if (c->counted) return SPH_OK;
     if (c->counts_dirty) {
         SPH_RUN(c, "clear_counts", launch_fill_u32(c->stream, c->d.count, c->cell_cap, 0u));
+        SPH_RUN(c, "clear_counts", launch_fill_u32(c->stream, c->d.bsum, (uint32_t)c->bsum_cap, 0u));
         c->counts_dirty = false;
     }
     SPH_RUN(c, "hash_count", launch_hash_count(c->stream, c->d));
     c->counted = true;
+    c->bsum_valid = false;
+    return SPH_OK;
+}
+
+// The scan's block sums of the counted B, summed from count[] unless the fused
+// step counted them.
+int ensure_block_sums(sph_ctx* c)
+{
+    if (c->bsum_valid) return SPH_OK;
+    SPH_RUN(c, "scan_reduce", launch_scan_reduce(c->stream, c->d));
+    c->bsum_valid = true;
     return SPH_OK;
 }
 
@@ -416,11 +433,12 @@ int enqueue_build(sph_ctx* c)
     const DeviceState& d = c->d;
     int rc = ensure_counted(c);
     if (rc != SPH_OK) return rc;
-    SPH_RUN(c, "scan_reduce", launch_scan_reduce(c->stream, d));
+    if ((rc = ensure_block_sums(c)) != SPH_OK) return rc;
     if (!scan_folds(d.total_cells)) SPH_RUN(c, "scan_top", launch_scan_top(c->stream, d));
     SPH_RUN(c, "scan_down", launch_scan_down(c->stream, d));  // also clears count[]
     c->counted = false;
-    SPH_RUN(c, "scatter", launch_scatter(c->stream, d));
+    SPH_RUN(c, "scatter", launch_scatter(c->stream, d));  // also clears bsum
+    c->bsum_valid = true;  // count[] and bsum both zero: the fused step counts both
     SPH_RUN(c, "rank_reorder", launch_rank_reorder(c->stream, d));
     return SPH_OK;
 }
@@ -448,20 +466,23 @@ int ensure_b(sph_ctx* c)
     return SPH_OK;
 }
 
-// Captures `steps` steps.  Every graph starts and ends with B counted (the
-// caller counts B before the first launch), so its first build has no hash;
-// capturing launches nothing, so the flags are restored afterwards.
+// Captures `steps` steps.  Every graph starts and ends with B counted and the
+// scan's block sums valid (the caller sees to both before the first launch),
+// so its first build has no hash and no reduction; capturing launches
+// nothing, so the flags are restored afterwards.
 int capture_graph(sph_ctx* c, int which, int steps)
 {
     hipGraph_t g = nullptr;
-    const bool counted = c->counted, dirty = c->counts_dirty;
+    const bool counted = c->counted, dirty = c->counts_dirty, bsum_valid = c->bsum_valid;
     c->counted = true;
+    c->bsum_valid = true;
     SPH_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
     int rc = SPH_OK;
     for (int k = 0; k < steps && rc == SPH_OK; ++k) rc = enqueue_step(c, k == steps - 1);
     hipError_t e = hipStreamEndCapture(c->stream, &g);
     c->counted = counted;
     c->counts_dirty = dirty;
+    c->bsum_valid = bsum_valid;
     if (rc != SPH_OK) {
         if (g) (void)hipGraphDestroy(g);
         return rc;
@@ -1027,7 +1048,8 @@ int sph_step(sph_ctx* c, int nsteps)
     int rc = set_device(c);
     if (rc != SPH_OK) return rc;
     if ((rc = ensure_b(c)) != SPH_OK) return rc;
-    if (!c->timing && (rc = ensure_counted(c)) != SPH_OK) return rc;  // graphs start from counted B
+    if (!c->timing && ((rc = ensure_counted(c)) != SPH_OK || (rc = ensure_block_sums(c)) != SPH_OK))
+        return rc;  // graphs start from counted B with its block sums
     if (nsteps == 0) {
         // Preparation only: the sorted, counted state (above) and both step graphs.
         if (!c->graph[1] && (rc = capture_graph(c, 1, kUnroll)) != SPH_OK) return rc;

@@ -47,7 +47,7 @@ struct DeviceState {
     uint32_t* tkey;   // scatter output: cell key
     uint32_t* count;  // per-cell counts (all zero between builds)
     uint32_t* start;  // per-cell first slot, total_cells+1 entries
-    uint32_t* bsum;   // scan block sums
+    uint32_t* bsum;   // scan block sums (counted by the fused step, else summed by k_scan_reduce)
     double* partials; // reduction scratch, 8 doubles per tile
 };
 

@@ -173,25 +173,39 @@ __global__ __launch_bounds__(kTile) void k_refresh(const Tile* __restrict__ tile
 // holding a run of equal keys (cell-sorted slots make runs the common case)
 // share one atomic.
 __device__ __forceinline__ void count_into_cell(uint32_t k, bool active, uint32_t s, uint32_t* __restrict__ slot,
-                                                uint32_t* __restrict__ count)
+                                                uint32_t* __restrict__ count, uint32_t* __restrict__ bsum)
 {
     const int lane = lane_id();
+    const uint64_t upto = (lane == 63) ? ~0ull : ((2ull << lane) - 1ull);  // bits 0..lane
+    // Lane `lane` heads a run [lane, next) of equal values of v; returns next.
+    auto run_end = [&](bool head) {
+        const uint64_t after = __ballot(head) & ~upto;
+        return after ? (__ffsll((long long)after) - 1) : kWave;
+    };
     const uint32_t prev = __shfl_up(k, 1u);
     const bool head = !active || lane == 0 || prev != k;
-    const uint64_t heads = __ballot(head);
-    const uint64_t upto = (lane == 63) ? ~0ull : ((2ull << lane) - 1ull);  // bits 0..lane
-    const int my_head = 63 - __clzll((long long)(heads & upto));
-    const uint64_t after = heads & ~upto;
-    const int next = after ? (__ffsll((long long)after) - 1) : kWave;
+    const int my_head = 63 - __clzll((long long)(__ballot(head) & upto));
+    const int next = run_end(head);
     uint32_t base = 0;
     if (head && active) base = atomicAdd(&count[k], (uint32_t)(next - lane));
     base = __shfl(base, my_head);
     if (active) slot[s] = base + (uint32_t)(lane - my_head);
+    // With bsum, the scan's block sums (kScanBlock cells each) are counted
+    // alongside, so the build needs no reduction pass over the cells.  Worth it
+    // only for cell-sorted slots, where a wave's keys fall in one or two blocks.
+    if (!bsum) return;
+    const uint32_t bk = k / (uint32_t)kScanBlock;
+    const uint32_t bprev = __shfl_up(bk, 1u);
+    const bool bhead = !active || lane == 0 || bprev != bk;
+    const int bnext = run_end(bhead);
+    if (bhead && active) atomicAdd(&bsum[bk], (uint32_t)(bnext - lane));
 }
 
 // Cell key of every slot of B, and its arrival rank inside the cell.  The
-// fused force+integrate kernel does the same for the state it writes, so a
-// step that follows one starts its build at the scan.
+// fused force+integrate kernel does the same for the state it writes, and
+// also counts the scan's block sums, so a step that follows one starts its
+// build at the scan.  Here B may be in any order (after an upload), where
+// per-particle block-sum atomics would contend, so k_scan_reduce sums them.
 __global__ __launch_bounds__(kTile) void k_hash_count(const Tile* __restrict__ tiles, const SceneDev* __restrict__ scenes,
                                                       const float4* __restrict__ P, uint32_t* __restrict__ key,
                                                       uint32_t* __restrict__ slot, uint32_t* __restrict__ count)
@@ -206,7 +220,7 @@ __global__ __launch_bounds__(kTile) void k_hash_count(const Tile* __restrict__ t
         k = global_key(sc, p.x, p.y, p.z);
         key[s] = k;
     }
-    count_into_cell(k, active, s, slot, count);
+    count_into_cell(k, active, s, slot, count, nullptr);
 }
 
 __global__ __launch_bounds__(kScanThreads) void k_scan_reduce(const uint32_t* __restrict__ count, uint32_t n,
@@ -304,11 +318,15 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_down(uint32_t* __restrict
     }
 }
 
+// Also clears the scan's block sums, which the scan has consumed, for the
+// next count.
 __global__ __launch_bounds__(kTile) void k_scatter(const Tile* __restrict__ tiles, const uint32_t* __restrict__ key,
                                                    const uint32_t* __restrict__ slot, const uint32_t* __restrict__ start,
                                                    const uint32_t* __restrict__ perm, uint32_t* __restrict__ torig,
-                                                   uint32_t* __restrict__ tsrc, uint32_t* __restrict__ tkey)
+                                                   uint32_t* __restrict__ tsrc, uint32_t* __restrict__ tkey,
+                                                   uint32_t* __restrict__ bsum, uint32_t nb)
 {
+    for (uint32_t i = blockIdx.x * (uint32_t)kTile + threadIdx.x; i < nb; i += gridDim.x * (uint32_t)kTile) bsum[i] = 0u;
     const Tile t = tiles[blockIdx.x];
     const uint32_t s = t.begin + threadIdx.x;
     if (s >= t.end) return;
@@ -481,7 +499,8 @@ __device__ __forceinline__ void force_epilogue(const SceneDev& sc, uint32_t s, b
                                                float rhoi, const uint32_t* __restrict__ perm,
                                                sph_particle_t* __restrict__ aos, float4* __restrict__ PN,
                                                float4* __restrict__ VN, uint32_t* __restrict__ key,
-                                               uint32_t* __restrict__ slot, uint32_t* __restrict__ count)
+                                               uint32_t* __restrict__ slot, uint32_t* __restrict__ count,
+                                               uint32_t* __restrict__ bsum)
 {
     float f[3];
     f[0] = fmaf(sc.pres_m, ps.ax, fmaf(sc.visc_m, ps.bx, rhoi * sc.grav[0]));
@@ -498,7 +517,7 @@ __device__ __forceinline__ void force_epilogue(const SceneDev& sc, uint32_t s, b
             k = global_key(sc, nx[0], nx[1], nx[2]);
             key[s] = k;
         }
-        count_into_cell(k, active, s, slot, count);  // the whole wave takes part
+        count_into_cell(k, active, s, slot, count, bsum);  // the whole wave takes part
         if (!active || !aos) return;  // no record, or one a later step of the same call overwrites
         float* r = reinterpret_cast<float*>(aos + perm[s]);
         r[0] = nx[0];
@@ -545,7 +564,8 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(SUB < 3 ?
                                                  const uint32_t* __restrict__ perm, sph_particle_t* __restrict__ aos,
                                                  float4* __restrict__ PN, float4* __restrict__ VN,
                                                  uint32_t* __restrict__ key, uint32_t* __restrict__ slot,
-                                                 uint32_t* __restrict__ count, uint32_t sentinel, uint32_t ntiles)
+                                                 uint32_t* __restrict__ count, uint32_t* __restrict__ bsum,
+                                                 uint32_t sentinel, uint32_t ntiles)
 {
     __shared__ uint32_t queue[kQueue][BLOCK];
     __shared__ uint32_t lds[BLOCK / 64][gather_lds_words<SUB>()];
@@ -635,7 +655,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(SUB < 3 ?
                                     }
                                 });
         flush(true);
-        force_epilogue<FUSE>(sc, s, active, ps, rhoi, perm, aos, PN, VN, key, slot, count);
+        force_epilogue<FUSE>(sc, s, active, ps, rhoi, perm, aos, PN, VN, key, slot, count, bsum);
     });
 }
 
@@ -826,7 +846,7 @@ hipError_t launch_scan_down(hipStream_t s, const DeviceState& d)
 hipError_t launch_scatter(hipStream_t s, const DeviceState& d)
 {
     hipLaunchKernelGGL(k_scatter, dim3(d.ntiles), dim3(kTile), 0, s, d.tiles, d.key, d.slot, d.start, d.perm, d.torig,
-                       d.tsrc, d.tkey);
+                       d.tsrc, d.tkey, d.bsum, (uint32_t)scan_blocks(d.total_cells));
     return hipGetLastError();
 }
 
@@ -862,11 +882,11 @@ hipError_t force_variant(hipStream_t s, const DeviceState& d, int mode)
     if (mode != (int)ForceMode::Pass)
         hipLaunchKernelGGL((k_force<true, SUB, PRUNE, BLOCK>), gather_grid(d), dim3(BLOCK), 0, s, d.gtiles, d.scenes,
                            d.start, d.PA, d.VA, d.rho, d.perm, mode == (int)ForceMode::FusedAos ? d.aos : nullptr,
-                           d.PB, d.VB, d.key, d.slot, d.count, d.n, (uint32_t)d.ngtiles);
+                           d.PB, d.VB, d.key, d.slot, d.count, d.bsum, d.n, (uint32_t)d.ngtiles);
     else
         hipLaunchKernelGGL((k_force<false, SUB, PRUNE, BLOCK>), gather_grid(d), dim3(BLOCK), 0, s, d.gtiles, d.scenes,
-                           d.start, d.PA, d.VA, d.rho, d.perm, d.aos, d.PB, d.VB, nullptr, nullptr, nullptr, d.n,
-                           (uint32_t)d.ngtiles);
+                           d.start, d.PA, d.VA, d.rho, d.perm, d.aos, d.PB, d.VB, nullptr, nullptr, nullptr, nullptr,
+                           d.n, (uint32_t)d.ngtiles);
     return hipGetLastError();
 }
 

@@ -151,6 +151,43 @@ def test_grid_bit_exact(pkg, orc, gpu_lib):
             assert_same_grid(start, perm, cstart, cperm, [g.ncells])
 
 
+def test_wide_box_grown_cells(pkg, orc, gpu_lib):
+    """A box 12 units wide and a few thousand particles: the default cell edge
+    would exceed the grid's cell budget (and the 2048-cell x axis, whose cells
+    are half as wide), so the edge grows. The grid equals the CPU build bit for
+    bit and each pass the O(N^2) FP64 transcription."""
+    lower, upper = (-6.0, -0.35, -0.1), (6.0, 1.0, 0.5)
+    rng = np.random.default_rng(3)
+    n = 3000
+    p = orc.particles(n)
+    centres = rng.uniform(-5.9, 5.9, 60)
+    p["pos"][:, 0] = (centres[rng.integers(0, 60, n)] + rng.normal(0.0, 0.01, n)).astype(np.float32)
+    p["pos"][:, 1] = rng.uniform(-0.30, -0.25, n).astype(np.float32)
+    p["pos"][:, 2] = rng.uniform(0.0, 0.05, n).astype(np.float32)
+    p["vel"][:, :3] = rng.normal(0.0, 0.3, (n, 3)).astype(np.float32)
+    b = pkg.sph.make_boundary(lower, upper)
+    ob = orc.make_boundary(lower, upper)
+    g = orc.grid_geometry(orc.smoothing_length(), ob, n, 2)
+    assert g.cs > float(orc.smoothing_length()) * (1 + 2 ** -10) / 2 * 1.1  # the edge grew
+    assert g.dims[0] <= 2048 and g.ncells <= 2 * max(2 ** 19, 16 * n)
+    with pkg.Simulation(n, boundaries=b) as sim:
+        sim.upload(p)
+        sim.rho_pres()
+        s1 = sim.download()
+        gi = sim.grid_info(0)
+        assert tuple(gi.dims) == tuple(g.dims) and gi.ncells == g.ncells
+        start, perm = sim.download_grid()
+        keys, cstart, cperm = orc.grid_build(p["pos"][:, :3], g)
+        assert_same_grid(start, perm, cstart, cperm, [g.ncells])
+        check_density(orc, p, s1, ob, grid=False)
+        sim.force()
+        s2 = sim.download()
+        check_force(orc, s1, s2, ob, grid=False)
+        sim.integrate()
+        s3 = sim.download()
+        check_integrate(orc, s2, s3, ob)
+
+
 TUNINGS = [(sub, prune, block, bpx) for sub in (1, 2, 3) for prune in (0, 1) for block in (128, 256)
            for bpx in (0, 3)]
 

@@ -556,9 +556,10 @@ def main(argv=None):
                     "valu_issue_slots": None if prof is None else prof.get("valu_issue_slots"),
                     "valu_lane_utilization": None if prof is None else prof.get("valu_lane_utilization"),
                     "density": None if prof is None else prof.get("density"),
-                    "binding_roof": "VALU issue and latency: the neighbour gathers are instruction-bound (candidate "
-                                    "scans and pair terms on the vector ALUs, 39% of lanes idle in divergent "
-                                    "loops), with HBM mostly idle"}
+                    "binding_roof": "latency of the per-candidate scan and per-neighbour drain loops: no unit "
+                                    "saturates (VALU busy about 64% with 37% of lanes idle in divergent loops, "
+                                    "texture data path about 78%), time follows the stencil candidate count, "
+                                    "and HBM is mostly idle"}
     # Host readback of the whole AoS (sph_download into pageable numpy memory):
     # what a frame would cost a caller that takes every step's particles to the
     # host instead of binding the device buffer.  Untimed region; not `value`.

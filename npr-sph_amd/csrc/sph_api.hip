@@ -43,14 +43,15 @@ struct GridGeom {
 // float32 rounding of the hash, |xi-xj| < h still lands within `sub` cells
 // along y and z and kXSplit*sub along x.  Grown by 2^(1/4) until each axis has
 // <= 2048 cells and the grid, z rounded up to whole bands of kCellBand planes,
-// has <= kXSplit * max(2^19, 16n) cells.
+// has <= min(kXSplit * max(2^19, 16n), 2^30) cells (global cell indices stay
+// below 2^31 for a batch of a few scenes of that size).
 bool grid_geometry(float h, const sph_boundary_t& b, uint64_t n, int sub, GridGeom& g)
 {
     if (!(h > 0.0f) || !std::isfinite(h) || sub < 1 || sub > 3) return false;
     for (int a = 0; a < 3; ++a)
         if (!std::isfinite(b.lower[a]) || !std::isfinite(b.upper[a])) return false;
     double cs = (double)h * (1.0 + 1.0 / 1024.0) / sub;
-    const uint64_t cap = (uint64_t)kXSplit * std::max<uint64_t>(524288u, 16u * n);
+    const uint64_t cap = std::min<uint64_t>((uint64_t)kXSplit * std::max<uint64_t>(524288u, 16u * n), 1ull << 30);
     for (;;) {
         uint64_t total = 1;
         bool ok = true;

@@ -80,8 +80,8 @@ static inline uint32_t orc_row(const orc_grid* g, int32_t y, int32_t z)
  * length, 1..3) and cs/ORC_XSPLIT along x: a particle within h of another is
  * at most `sub` cells away along y and z and ORC_XSPLIT*sub along x despite
  * float32 rounding of the hash.  The edge grows by 2^(1/4) steps until every
- * axis has <= 2048 cells and the banded grid has <= ORC_XSPLIT*max(2^19, 16n)
- * cells. */
+ * axis has <= 2048 cells and the banded grid has
+ * <= min(ORC_XSPLIT*max(2^19, 16n), 2^30) cells. */
 int orc_grid_geometry(float h, const orc_boundary* b, uint64_t n, int sub, orc_grid* g)
 {
     if (!(h > 0.0f) || !isfinite(h) || sub < 1 || sub > 3) return -1;
@@ -91,6 +91,7 @@ int orc_grid_geometry(float h, const orc_boundary* b, uint64_t n, int sub, orc_g
     uint64_t cap = 16u * n;
     if (cap < 524288u) cap = 524288u;
     cap *= ORC_XSPLIT;
+    if (cap > (1ull << 30)) cap = 1ull << 30;
     for (;;) {
         uint64_t total = 1;
         int ok = 1;

@@ -294,6 +294,9 @@ def test_grid_geometry_rule(orc):
         assert gg.ncells == gg.dims[0] * gg.dims[1] * 8 * -(-gg.dims[2] // 8)
     # neighbour safety: cell edge * 2 >= h always
     assert 2 * g2.cs >= float(h)
+    # a huge, sparse scene: the cell budget stops at 2^30 (cell indices fit 31 bits)
+    g4 = orc.grid_geometry(h, orc.make_boundary((-40, -40, -40), (40, 40, 40)), 200_000_000)
+    assert g4.ncells <= 2 ** 30 and max(g4.dims) <= 2048
     # every axis covers the box: x in cells of cs/2, y and z in cells of cs
     for gg, bb in ((g, orc.default_boundary()), (g2, big)):
         ext = [float(bb.upper[a]) - float(bb.lower[a]) for a in range(3)]

@@ -1548,9 +1548,13 @@ int sph_coarse_step(sph_ctx* c, int n, const sph_coarse_opts_t* o, sph_coarse_st
     bool rejudge = false;
     for (int k = 0; k < n; ++k) {
         if (cooldown > 0) {
-            --cooldown;
-            if ((rc = sph_step(c, 4)) != SPH_OK) return rc;
-            s.plain_steps += 4;
+            // The cooldown's coarse steps (up to the call's end) as one call of
+            // 4 plain steps each: the same trajectory in 16-step graphs.
+            const int run = std::min(cooldown, n - k);
+            cooldown -= run;
+            k += run - 1;
+            if ((rc = sph_step(c, 4 * run)) != SPH_OK) return rc;
+            s.plain_steps += 4 * run;
             rejudge = cooldown == 0;
             continue;
         }

@@ -97,6 +97,8 @@ typedef struct sph_lattice {
 
 /* Cells are SPH_GRID_X_SPLIT times narrower along x than along y and z. */
 #define SPH_GRID_X_SPLIT 2
+/* Cells are keyed in bands of SPH_GRID_BAND z-planes (see sph_grid_info_t.ncells). */
+#define SPH_GRID_BAND 8
 
 /* Geometry of one scene's uniform neighbour grid (read-only diagnostics).
  * Cell coordinate along an axis = clamp(floor((p - lo) * inv), 0, dims - 1) in
@@ -108,9 +110,9 @@ typedef struct sph_grid_info {
     double cell;          /* cell edge along y and z (>= smoothing length / cells_per_h);
                              cell / SPH_GRID_X_SPLIT along x */
     int32_t dims[3];      /* cells per axis */
-    uint32_t ncells;      /* dims[0]*dims[1]*8*ceil(dims[2]/8): cells are keyed band by band,
-                             key = ((z/8*dims[1] + y)*8 + z%8)*dims[0] + x; cell `ncells`
-                             holds NaN positions */
+    uint32_t ncells;      /* dims[0]*dims[1]*B*ceil(dims[2]/B), B = SPH_GRID_BAND: cells are keyed
+                             band by band, key = ((z/B*dims[1] + y)*B + z%B)*dims[0] + x;
+                             cell `ncells` holds NaN positions */
     uint32_t cell_base;   /* global index of this scene's first cell */
     uint32_t first;       /* first particle of this scene in the AoS buffer */
     uint32_t count;       /* particles in this scene */

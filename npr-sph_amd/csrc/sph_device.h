@@ -52,14 +52,14 @@ static_assert(sizeof(SceneDev) == 128, "SceneDev layout");
 // not; and the eighths of the slots that the XCDs take are still slabs of z,
 // each holding full columns of the fluid's depth.  The last band of a scene
 // may be partial; its missing planes are cells that stay empty.
-constexpr int kCellBand = 8;
+constexpr int kCellBand = SPH_GRID_BAND;
 
 // Cells are kXSplit times narrower along x than along y and z.  A stencil row
 // (fixed y, z) is a contiguous slot range whatever the x edge, so narrower x
 // cells trim the chord ends of each row (fewer candidates) without adding
 // rows; only the cell count grows.  A particle's stencil reaches kXSplit*sub
 // cells along x and sub cells along y and z.
-constexpr int kXSplit = 2;
+constexpr int kXSplit = SPH_GRID_X_SPLIT;
 
 struct alignas(16) Tile {
     uint32_t scene;

@@ -51,14 +51,20 @@ def test_struct_layouts(pkg):
     assert s.PARTICLE_DTYPE.fields["pos"][1] == 0 and s.PARTICLE_DTYPE.fields["extras"][1] == 48
 
 
-def test_grid_x_split_constant_agrees(pkg):
-    """The x split of the cell grid is declared once in the header; the Python
-    binding and the bench's candidate counter use the same value."""
+def test_grid_layout_constants_agree(pkg):
+    """The x split and the z-band height of the cell grid are declared once in
+    the header; the Python binding, the bench's candidate counter and the
+    oracle's CPU grid build use the same values."""
     hdr = open(os.path.join(ROOT, "include", "sph.h")).read()
-    m = re.search(r"#define SPH_GRID_X_SPLIT (\d+)", hdr)
-    assert m and int(m.group(1)) == pkg.sph.GRID_X_SPLIT
+    split = re.search(r"#define SPH_GRID_X_SPLIT (\d+)", hdr).group(1)
+    band = re.search(r"#define SPH_GRID_BAND (\d+)", hdr).group(1)
+    assert int(split) == pkg.sph.GRID_X_SPLIT
     bench = open(os.path.join(ROOT, "bench.py")).read()
-    assert re.search(r"^X_SPLIT = (\d+)$", bench, re.M).group(1) == m.group(1)
+    assert re.search(r"^X_SPLIT = (\d+)$", bench, re.M).group(1) == split
+    assert re.search(r"^CELL_BAND = (\d+)$", bench, re.M).group(1) == band
+    oracle = open(os.path.join(ROOT, "oracle", "sph_oracle.c")).read()
+    assert re.search(r"#define ORC_XSPLIT (\d+)", oracle).group(1) == split
+    assert re.search(r"#define ORC_BAND (\d+)", oracle).group(1) == band
 
 
 def test_defaults_are_the_references(pkg, gpu_lib):

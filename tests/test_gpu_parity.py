@@ -188,6 +188,48 @@ def test_wide_box_grown_cells(pkg, orc, gpu_lib):
         check_integrate(orc, s2, s3, ob)
 
 
+@pytest.mark.parametrize("case", range(8))
+def test_random_boxes_vs_oracle(pkg, orc, gpu_lib, case):
+    """Seeded random scenes: boxes from thin to wide, particles in clusters
+    and some beyond the walls, every cells_per_h. The grid equals the CPU
+    build bit for bit and each pass the O(N^2) FP64 transcription."""
+    rng = np.random.default_rng(1000 + case)
+    h = float(orc.smoothing_length())
+    ext = rng.uniform([0.05, 0.05, 0.05], [3.0, 1.5, 3.0])
+    lo = rng.uniform(-1.0, 1.0, 3)
+    lower, upper = tuple(lo), tuple(lo + ext)
+    n = int(rng.integers(500, 2500))
+    p = orc.particles(n)
+    nc = int(rng.integers(1, 12))
+    centres = lo + rng.uniform(0.0, 1.0, (nc, 3)) * ext
+    pos = centres[rng.integers(0, nc, n)] + rng.normal(0.0, 0.6 * h, (n, 3))
+    out = rng.random(n) < 0.02  # a few beyond the walls
+    pos[out] += rng.normal(0.0, 0.2, (int(out.sum()), 3))
+    p["pos"][:, :3] = pos.astype(np.float32)
+    p["vel"][:, :3] = rng.normal(0.0, 0.5, (n, 3)).astype(np.float32)
+    sub = 1 + case % 3
+    b = pkg.sph.make_boundary(lower, upper)
+    ob = orc.make_boundary(lower, upper)
+    with pkg.Simulation(n, boundaries=b) as sim:
+        sim.set_tuning(cells_per_h=sub, prune=case % 2, block=128 if case < 4 else 256)
+        sim.upload(p)
+        sim.rho_pres()
+        s1 = sim.download()
+        g = orc.grid_geometry(orc.smoothing_length(), ob, n, sub)
+        gi = sim.grid_info(0)
+        assert tuple(gi.dims) == tuple(g.dims) and gi.ncells == g.ncells, (case, tuple(gi.dims), tuple(g.dims))
+        start, perm = sim.download_grid()
+        keys, cstart, cperm = orc.grid_build(p["pos"][:, :3], g)
+        assert_same_grid(start, perm, cstart, cperm, [g.ncells])
+        check_density(orc, p, s1, ob, grid=False)
+        sim.force()
+        s2 = sim.download()
+        check_force(orc, s1, s2, ob, grid=False)
+        sim.integrate()
+        s3 = sim.download()
+        check_integrate(orc, s2, s3, ob)
+
+
 TUNINGS = [(sub, prune, block, bpx) for sub in (1, 2, 3) for prune in (0, 1) for block in (128, 256)
            for bpx in (0, 3)]
 

@@ -1549,8 +1549,9 @@ int sph_coarse_step(sph_ctx* c, int n, const sph_coarse_opts_t* o, sph_coarse_st
     for (int k = 0; k < n; ++k) {
         if (cooldown > 0) {
             // The cooldown's coarse steps (up to the call's end) as one call of
-            // 4 plain steps each: the same trajectory in 16-step graphs.
-            const int run = std::min(cooldown, n - k);
+            // 4 plain steps each: the same trajectory in 16-step graphs.  At most
+            // 2^28 coarse steps per call, so 4*run fits an int.
+            const int run = std::min({cooldown, n - k, 1 << 28});
             cooldown -= run;
             k += run - 1;
             if ((rc = sph_step(c, 4 * run)) != SPH_OK) return rc;

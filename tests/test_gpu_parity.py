@@ -360,6 +360,35 @@ def test_dam8m_one_step_vs_grid_oracle(pkg, orc, gpu_lib):
         assert_same_grid(start, perm, cstart, cperm, [g.ncells])
 
 
+def test_dam64m_cells_beyond_2e28_vs_grid_oracle(pkg, orc, gpu_lib):
+    """64M particles in dam1m's box scaled 4x per axis: 321M cells, so cell keys
+    and the cell-start array run past 2^28 and the scan folds 150k block sums.
+    Three steps from the lattice, then the density and force passes against the
+    FP64 grid transcription and the grid bit for bit against the CPU build."""
+    sc = pkg.scenes.dam_block(400, 4.0 * pkg.scenes.box_scale(1_000_000))
+    b = pkg.sph.make_boundary(sc.lower, sc.upper)
+    ob = orc.make_boundary(sc.lower, sc.upper)
+    print(f"dam64m: {sc.count} particles", flush=True)
+    with pkg.Simulation(sc.count, boundaries=b) as sim:
+        sim.init_lattice(pkg.sph.make_lattice(**sc.lattice_args()))
+        sim.step(3)
+        st = sim.download()
+        sim.rho_pres()
+        s1 = sim.download()
+        check_density(orc, st, s1, ob, grid=True)
+        del st
+        sim.force()
+        s2 = sim.download()
+        check_force(orc, s1, s2, ob, grid=True)
+        del s1
+        gi = sim.grid_info(0)
+        g = orc.grid_geometry(orc.smoothing_length(), ob, sc.count, gi.cells_per_h)
+        assert gi.ncells == g.ncells and g.ncells > 2 ** 28
+        start, perm = sim.download_grid()
+        _, cstart, cperm = orc.grid_build(s2["pos"][:, :3], g)
+        assert_same_grid(start, perm, cstart, cperm, [g.ncells])
+
+
 def test_reference_make_grid_scene(pkg, orc, gpu_lib):
     """The reference's literal initial state (make_grid, Main.cpp:488-505: 10x100x10
     at spacing PARTICLE_RADIUS = h/4, ~251 neighbours) through all three passes,

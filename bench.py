@@ -566,7 +566,9 @@ def main(argv=None):
                     "binding_roof": "latency of the per-candidate scan and per-neighbour drain loops: no unit "
                                     "saturates (VALU busy about 64% with 37% of lanes idle in divergent loops, "
                                     "texture data path about 78%), time follows the stencil candidate count, "
-                                    "and HBM is mostly idle"}
+                                    "and HBM is mostly idle; at 1M each wave slot runs only about three "
+                                    "50-300 us waves per launch, so the launch tail leaves slots idle "
+                                    "(71% slot use for force, 62% for density, against 89% at 8M)"}
     # Host readback of the whole AoS (sph_download into pageable numpy memory):
     # what a frame would cost a caller that takes every step's particles to the
     # host instead of binding the device buffer.  Untimed region; not `value`.

@@ -1185,8 +1185,12 @@ int sph_metrics_noalloc(sph_ctx* c, int scene, sph_metrics_t* out)
     if (!c->scratch) return fail(c, SPH_ERR_STATE, "allocation-free scratch not resolved for this device");
     double* partials = c->scratch;
     double* folded = c->scratch + (size_t)kNoAllocTiles * 8u;
-    SPH_RUN(c, "metrics", launch_metrics(c->stream, c->d, t0, nt, partials));
-    SPH_RUN(c, "metrics_fold", launch_metrics_fold(c->stream, partials, nt, folded));
+    // Launched directly, not through the kernel-timing wrapper, whose events
+    // may be created on demand.
+    if (const hipError_t e = launch_metrics(c->stream, c->d, t0, nt, partials); e != hipSuccess)
+        return hip_fail(c, e, "metrics");
+    if (const hipError_t e = launch_metrics_fold(c->stream, partials, nt, folded); e != hipSuccess)
+        return hip_fail(c, e, "metrics_fold");
     double acc[8];
     SPH_HIP(c, hipMemcpyAsync(acc, folded, sizeof(double) * 7u, hipMemcpyDeviceToHost, c->stream));
     SPH_HIP(c, hipStreamSynchronize(c->stream));
